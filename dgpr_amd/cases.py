@@ -1,0 +1,94 @@
+"""Workload builders: meshes, reference states and initial states of the
+benchmark and parity cases.
+
+All inputs are synthetic and seeded (splitmix64 streams inside libdgx), so
+the same call produces bit-identical meshes and states on any host.
+
+Physics follows the reference's semantics: no heat conduction and an
+adiabatic-energy no-slip wall state (flux.cpp:156-202, bound_states.cpp:42-49);
+Pr only names the case.  The reference has only a fixed time step; the
+channel's dt is derived here from CFL 0.3 as
+    dt = CFL * h_min / ((|u|_max + c) (2p + 1)).
+"""
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import dgx
+
+TWO_PI = 2.0 * math.pi
+
+
+@dataclass
+class Case:
+    name: str
+    mesh: "dgx.Mesh"
+    params: "dgx.Params"
+    U0: np.ndarray
+    scheme: str
+    dt: float
+    nsteps: int
+
+    @property
+    def dof(self):
+        return self.mesh.nelem * self.mesh.nbf
+
+    @property
+    def stages(self):
+        return 4 if self.scheme == "RK4" else 3
+
+
+def periodic_bounds():
+    return [dgx.PERIODIC] * 6
+
+
+def tgv_case(n=8, p=2, seed=0, noise=1e-3, M=0.1, Re=1600.0, rho0=1.0, V0=1.0, L=1.0,
+             dt=1e-3, nsteps=10, scheme="RK4", gamma=1.4, eta=6.0, nxyz=None, amp=0.0):
+    """Taylor-Green-type state on the periodic cube [0, 2pi]^3 (q = 1).
+
+    ``amp`` > 0 displaces interior nodes smoothly (non-affine elements) while
+    keeping the periodic faces matched."""
+    nx, ny, nz = nxyz if nxyz is not None else (n, n, n)
+    nodes = dgx.box_nodes(1, nx, ny, nz, 0.0, 0.0, 0.0, TWO_PI, TWO_PI, TWO_PI, amp, seed)
+    mesh = dgx.Mesh(p, 1, nx, ny, nz, periodic_bounds(), nodes)
+    c = V0 / M
+    mu = rho0 * V0 * L / Re
+    prm = dgx.make_reference_state(gamma=gamma, R=287.0, mu=mu, c=c, rho=rho0, u=0.0, v=0.0,
+                                   w=0.0, eta=eta)
+    U0 = dgx.tgv_state(mesh, gamma, rho0, V0, prm.P, noise, seed)
+    return Case("tgv_%dx%dx%d_p%d" % (nx, ny, nz, p), mesh, prm, U0, scheme, dt, nsteps)
+
+
+def channel_case(nx=48, ny=24, nz=24, p=3, q=1, seed=1, amp=0.1, M=0.2, Re=3000.0, rho0=1.0,
+                 Uc=1.0, frac=0.05, kmax=4, cfl=0.3, nsteps=200, scheme="tvdRK3",
+                 gamma=1.4, eta=6.0, wall=dgx.NO_SLIP_WALL):
+    """Channel [0,4pi]x[-1,1]x[0,4pi/3], periodic in x and z, walls at y=+-1,
+    interior nodes displaced by a smooth field of amplitude amp*h."""
+    lx, ly, lz = 4.0 * math.pi, 2.0, 4.0 * math.pi / 3.0
+    nodes = dgx.box_nodes(q, nx, ny, nz, 0.0, -1.0, 0.0, lx, ly, lz, amp, seed)
+    bounds = [dgx.PERIODIC, dgx.PERIODIC, wall, wall, dgx.PERIODIC, dgx.PERIODIC]
+    mesh = dgx.Mesh(p, q, nx, ny, nz, bounds, nodes)
+    c = Uc / M
+    mu = rho0 * Uc * 1.0 / Re
+    prm = dgx.make_reference_state(gamma=gamma, R=287.0, mu=mu, c=c, rho=rho0, u=0.0, v=0.0,
+                                   w=0.0, eta=eta)
+    U0 = dgx.channel_state(mesh, gamma, rho0, Uc, prm.P, frac, kmax, lx, lz, seed)
+    hmin = min(lx / nx, ly / ny, lz / nz)
+    dt = cfl * hmin / ((Uc + c) * (2 * p + 1))
+    return Case("channel_%dx%dx%d_p%d" % (nx, ny, nz, p), mesh, prm, U0, scheme, dt, nsteps)
+
+
+def config0():
+    """8^3 periodic p=2 Taylor-Green, RK4 dt=1e-3, 10 steps."""
+    return tgv_case(n=8, p=2, seed=0, dt=1e-3, nsteps=10, scheme="RK4")
+
+
+def config1():
+    """48x24x24 p=3 curved channel with no-slip walls, TVD-RK3, 200 steps."""
+    return channel_case()
+
+
+def config2():
+    """64^3 periodic p=3 Taylor-Green, RK4 dt=5e-4, 100 steps."""
+    return tgv_case(n=64, p=3, seed=2, dt=5e-4, nsteps=100, scheme="RK4")

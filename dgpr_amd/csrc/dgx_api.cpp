@@ -1,0 +1,474 @@
+// dgx_api.cpp -- C ABI of libdgx: device context, state transfer, residual
+// evaluation and RK stage sequencing on one HIP stream.
+//
+// Stage algebra follows the CPU drivers RK4_distributed and
+// tvdRK3_distributed (time_distributed.cpp:46-102), including classic RK4
+// (stage 3 uses U + dt f3) and the RMS residual norm of the first stage
+// (time_distributed.cpp:23-44).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dgx.h"
+#include "dgx_launch.hpp"
+
+using namespace dgx;
+
+struct dgx_ctx
+{
+  int device;
+  int p, q;
+  int nx, ny, nz, nelem, nbf;
+  int bounds[6];
+  size_t nstate;  // nelem*5*nbf
+  dgx_params params;
+  Phys phys;
+  hipStream_t stream;
+  double* nodes;    // [e][(q+1)^3][3]
+  double* U;        // state, Gauss-nodal basis
+  double* Ua;       // stage buffer / staging
+  double* Ub;       // stage buffer
+  double* acc;      // RK4 accumulator
+  double* fs;       // f_g scratch
+  double* partial;  // per-block sums of R^2
+  double* sum;      // reduced sum
+  int* status;      // kernel-side error flags
+  int npartial;
+  size_t bytes;
+  char err[512];
+};
+
+static thread_local char g_create_err[512] = "";
+static std::mutex g_tab_mutex;
+static std::vector<int> g_tab_devices;
+
+static dgx_status fail(dgx_ctx* c, dgx_status st, const char* fmt, ...)
+{
+  char* buf = c ? c->err : g_create_err;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, 512, fmt, ap);
+  va_end(ap);
+  return st;
+}
+
+#define HIPCHK(ctx, call)                                                                  \
+  do                                                                                       \
+  {                                                                                        \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail((ctx), DGX_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
+  } while (0)
+
+static void derive_phys(const dgx_params& p, Phys& ph)
+{
+  // total conditions of the reference state (solve.cpp:167-175)
+  double gm1 = p.gamma - 1.0;
+  double u = p.Ufr[1] / p.Ufr[0], v = p.Ufr[2] / p.Ufr[0], w = p.Ufr[3] / p.Ufr[0];
+  double M = sqrt(u * u + v * v + w * w) / p.c;
+  ph.gamma = p.gamma;
+  ph.mu    = p.mu;
+  ph.eta   = p.eta;
+  ph.Rgas  = p.R;
+  ph.P     = p.P;
+  ph.Tt    = p.T * (1.0 + 0.5 * gm1 * M * M);
+  ph.Pt    = p.P * pow(ph.Tt / p.T, p.gamma / gm1);
+  for (int i = 0; i < 5; ++i)
+  {
+    ph.Ufr[i]    = p.Ufr[i];
+    ph.source[i] = p.source[i];
+  }
+}
+
+static MeshArgs mesh_args(const dgx_ctx* c)
+{
+  MeshArgs m;
+  m.nodes = c->nodes;
+  m.nx = c->nx;
+  m.ny = c->ny;
+  m.nz = c->nz;
+  m.nelem = c->nelem;
+  for (int i = 0; i < 6; ++i)
+    m.bounds[i] = c->bounds[i];
+  return m;
+}
+
+static dgx_status ensure_tables(int device)
+{
+  std::lock_guard<std::mutex> lk(g_tab_mutex);
+  for (int d : g_tab_devices)
+    if (d == device)
+      return DGX_OK;
+  static PTab pt[3];
+  static QTab qt[9];
+  static bool built = false;
+  if (!built)
+  {
+    host::build_tables(pt, qt);
+    built = true;
+  }
+  hipError_t e = upload_tables(pt, qt);
+  if (e != hipSuccess)
+    return fail(nullptr, DGX_ERR_HIP, "table upload failed: %s", hipGetErrorString(e));
+  g_tab_devices.push_back(device);
+  return DGX_OK;
+}
+
+extern "C" {
+
+const char* dgx_last_error(const dgx_ctx* ctx)
+{
+  return ctx ? ctx->err : g_create_err;
+}
+
+dgx_status dgx_make_reference_state(double gamma, double R, double mu, double c, double rho,
+                                    double u, double v, double w, double eta,
+                                    const double* source, dgx_params* out)
+{
+  if (!out || rho <= 0.0 || gamma <= 1.0 || R <= 0.0)
+    return DGX_ERR_INVALID;
+  double P = c * c * (rho / gamma);
+  out->gamma = gamma;
+  out->R = R;
+  out->mu = mu;
+  out->c = c;
+  out->T = P / (rho * R);
+  out->P = P;
+  out->eta = eta;
+  out->Ufr[0] = rho;
+  out->Ufr[1] = rho * u;
+  out->Ufr[2] = rho * v;
+  out->Ufr[3] = rho * w;
+  out->Ufr[4] = (P / (gamma - 1.0)) + 0.5 * rho * (u * u + v * v + w * w);
+  for (int i = 0; i < 5; ++i)
+    out->source[i] = source ? source[i] : 0.0;
+  return DGX_OK;
+}
+
+dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** out)
+{
+  if (!mesh || !params || !out || !mesh->nodes)
+    return fail(nullptr, DGX_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (mesh->p < 1 || mesh->p > 3)
+    return fail(nullptr, DGX_ERR_UNSUPPORTED, "solution order p=%llu outside 1..3",
+                (unsigned long long)mesh->p);
+  if (mesh->q < 1 || mesh->q > 3)
+    return fail(nullptr, DGX_ERR_UNSUPPORTED, "geometry order q=%llu outside 1..3",
+                (unsigned long long)mesh->q);
+  if (mesh->nx == 0 || mesh->ny == 0 || mesh->nz == 0)
+    return fail(nullptr, DGX_ERR_INVALID, "empty mesh");
+  unsigned long long ne = (unsigned long long)mesh->nx * mesh->ny * mesh->nz;
+  if (ne > (1ull << 30))
+    return fail(nullptr, DGX_ERR_UNSUPPORTED, "mesh of %llu elements too large", ne);
+  for (int d = 0; d < 3; ++d)
+  {
+    if ((mesh->bounds[2 * d] == DGX_BT_PERIODIC) != (mesh->bounds[2 * d + 1] == DGX_BT_PERIODIC))
+      return fail(nullptr, DGX_ERR_INVALID, "periodic boundaries are not matching");
+    for (int s = 0; s < 2; ++s)
+    {
+      int64_t b = mesh->bounds[2 * d + s];
+      if (b > -2 || b < -7)
+        return fail(nullptr, DGX_ERR_INVALID, "invalid boundary code %lld", (long long)b);
+    }
+  }
+  if (params->Ufr[0] <= 0.0 || params->gamma <= 1.0)
+    return fail(nullptr, DGX_ERR_INVALID, "invalid reference state");
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, DGX_ERR_NODEVICE, "no HIP device visible");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return fail(nullptr, DGX_ERR_HIP, "hipGetDevice failed");
+  dgx_status st = ensure_tables(dev);
+  if (st != DGX_OK)
+    return st;
+
+  dgx_ctx* c = new dgx_ctx();
+  memset(c, 0, sizeof(*c));
+  c->device = dev;
+  c->p = (int)mesh->p;
+  c->q = (int)mesh->q;
+  c->nx = (int)mesh->nx;
+  c->ny = (int)mesh->ny;
+  c->nz = (int)mesh->nz;
+  c->nelem = (int)ne;
+  c->nbf = (c->p + 1) * (c->p + 1) * (c->p + 1);
+  for (int i = 0; i < 6; ++i)
+    c->bounds[i] = (int)mesh->bounds[i];
+  c->nstate = (size_t)c->nelem * 5 * c->nbf;
+  c->params = *params;
+  derive_phys(*params, c->phys);
+  size_t nn = (size_t)c->nelem * (c->q + 1) * (c->q + 1) * (c->q + 1) * 3;
+  c->npartial = resid_out_blocks(c->p, c->nelem);
+
+  auto bail = [&](const char* what, hipError_t e) {
+    snprintf(g_create_err, sizeof(g_create_err), "%s failed: %s", what, hipGetErrorString(e));
+    dgx_destroy(c);
+    return DGX_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+    return bail("hipStreamCreate", e);
+  size_t sb = c->nstate * sizeof(double);
+  double** bufs[5] = {&c->U, &c->Ua, &c->Ub, &c->acc, &c->fs};
+  for (int i = 0; i < 5; ++i)
+    if ((e = hipMalloc(bufs[i], sb)) != hipSuccess)
+      return bail("hipMalloc(state)", e);
+  if ((e = hipMalloc(&c->nodes, nn * sizeof(double))) != hipSuccess)
+    return bail("hipMalloc(nodes)", e);
+  if ((e = hipMalloc(&c->partial, (size_t)c->npartial * sizeof(double))) != hipSuccess)
+    return bail("hipMalloc(partial)", e);
+  if ((e = hipMalloc(&c->sum, sizeof(double))) != hipSuccess)
+    return bail("hipMalloc(sum)", e);
+  if ((e = hipMalloc(&c->status, sizeof(int))) != hipSuccess)
+    return bail("hipMalloc(status)", e);
+  c->bytes = 5 * sb + nn * sizeof(double) + (size_t)c->npartial * sizeof(double) + 16;
+  if ((e = hipMemcpyAsync(c->nodes, mesh->nodes, nn * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream)) != hipSuccess)
+    return bail("hipMemcpy(nodes)", e);
+  if ((e = hipMemsetAsync(c->U, 0, sb, c->stream)) != hipSuccess)
+    return bail("hipMemset(U)", e);
+  if ((e = hipMemsetAsync(c->status, 0, sizeof(int), c->stream)) != hipSuccess)
+    return bail("hipMemset(status)", e);
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
+    return bail("hipStreamSynchronize", e);
+  *out = c;
+  return DGX_OK;
+}
+
+dgx_status dgx_destroy(dgx_ctx* c)
+{
+  if (!c)
+    return DGX_OK;
+  if (c->stream)
+    (void)hipStreamSynchronize(c->stream);
+  double* bufs[8] = {c->U, c->Ua, c->Ub, c->acc, c->fs, c->nodes, c->partial, c->sum};
+  for (double* b : bufs)
+    if (b)
+      (void)hipFree(b);
+  if (c->status)
+    (void)hipFree(c->status);
+  if (c->stream)
+    (void)hipStreamDestroy(c->stream);
+  delete c;
+  return DGX_OK;
+}
+
+dgx_status dgx_set_params(dgx_ctx* c, const dgx_params* params)
+{
+  if (!c || !params)
+    return DGX_ERR_INVALID;
+  if (params->Ufr[0] <= 0.0 || params->gamma <= 1.0)
+    return fail(c, DGX_ERR_INVALID, "invalid reference state");
+  c->params = *params;
+  derive_phys(*params, c->phys);
+  return DGX_OK;
+}
+
+dgx_status dgx_get_sizes(const dgx_ctx* c, uint64_t out[4])
+{
+  if (!c || !out)
+    return DGX_ERR_INVALID;
+  out[0] = (uint64_t)c->nelem;
+  out[1] = (uint64_t)c->nbf;
+  out[2] = (uint64_t)c->nstate;
+  out[3] = (uint64_t)c->bytes;
+  return DGX_OK;
+}
+
+dgx_status dgx_get_stream(dgx_ctx* c, void** stream)
+{
+  if (!c || !stream)
+    return DGX_ERR_INVALID;
+  *stream = (void*)c->stream;
+  return DGX_OK;
+}
+
+dgx_status dgx_sync(dgx_ctx* c)
+{
+  if (!c)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
+dgx_status dgx_set_state_device(dgx_ctx* c, const double* U_dev)
+{
+  if (!c || !U_dev)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, basis_dispatch(c->p, U_dev, c->U, c->nelem, 0, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
+dgx_status dgx_get_state_device(dgx_ctx* c, double* U_dev)
+{
+  if (!c || !U_dev)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, basis_dispatch(c->p, c->U, U_dev, c->nelem, 1, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
+dgx_status dgx_set_state(dgx_ctx* c, const double* U_host)
+{
+  if (!c || !U_host)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, hipMemcpyAsync(c->Ua, U_host, c->nstate * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, basis_dispatch(c->p, c->Ua, c->U, c->nelem, 0, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
+dgx_status dgx_get_state(dgx_ctx* c, double* U_host)
+{
+  if (!c || !U_host)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, basis_dispatch(c->p, c->U, c->Ua, c->nelem, 1, c->stream));
+  HIPCHK(c, hipMemcpyAsync(U_host, c->Ua, c->nstate * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
+static hipError_t rhs(dgx_ctx* c, const double* Vin, const double* Vbase, double* acc,
+                      double* Vout, double* fout, double ca, double cb, double cf, int accmode)
+{
+  StageArgs sa;
+  sa.Vin = Vin;
+  sa.Vbase = Vbase;
+  sa.acc = acc;
+  sa.Vout = Vout;
+  sa.fout = fout;
+  sa.ca = ca;
+  sa.cb = cb;
+  sa.cf = cf;
+  sa.accmode = accmode;
+  return rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->status, c->stream);
+}
+
+static dgx_status check_status(dgx_ctx* c)
+{
+  int h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h)
+  {
+    (void)hipMemsetAsync(c->status, 0, sizeof(int), c->stream);
+    return fail(c, DGX_ERR_INVALID, "negative inflow boundary Mach number");
+  }
+  return DGX_OK;
+}
+
+dgx_status dgx_residual_device(dgx_ctx* c, const double* U_dev, double* R_dev, double* f_dev)
+{
+  if (!c || !U_dev || !f_dev)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, basis_dispatch(c->p, U_dev, c->Ua, c->nelem, 0, c->stream));
+  HIPCHK(c, rhs(c, c->Ua, nullptr, nullptr, nullptr, c->fs, 0, 0, 0, 0));
+  HIPCHK(c, resid_out_dispatch(c->p, c->q, mesh_args(c), c->fs, f_dev, R_dev, nullptr,
+                               c->stream));
+  return check_status(c);
+}
+
+dgx_status dgx_residual(dgx_ctx* c, double* R_host, double* f_host)
+{
+  if (!c || !f_host)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, rhs(c, c->U, nullptr, nullptr, nullptr, c->fs, 0, 0, 0, 0));
+  HIPCHK(c, resid_out_dispatch(c->p, c->q, mesh_args(c), c->fs, c->Ua, R_host ? c->Ub : nullptr,
+                               nullptr, c->stream));
+  size_t sb = c->nstate * sizeof(double);
+  HIPCHK(c, hipMemcpyAsync(f_host, c->Ua, sb, hipMemcpyDeviceToHost, c->stream));
+  if (R_host)
+    HIPCHK(c, hipMemcpyAsync(R_host, c->Ub, sb, hipMemcpyDeviceToHost, c->stream));
+  return check_status(c);
+}
+
+// Launches one step; if want_norm, stores sum(R^2) of stage 1 in c->sum.
+static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm)
+{
+  hipError_t e;
+  if (scheme == DGX_RK4)
+  {
+    // U1 = U + dt/2 f1 ; acc = f1
+    if ((e = rhs(c, c->U, c->U, c->acc, c->Ua, nullptr, 1.0, 0.0, 0.5 * dt, 1)) != hipSuccess)
+      return e;
+    if (want_norm)
+    {
+      if ((e = resid_out_dispatch(c->p, c->q, mesh_args(c), c->acc, nullptr, nullptr,
+                                  c->partial, c->stream)) != hipSuccess)
+        return e;
+      if ((e = sum_dispatch(c->partial, c->npartial, c->sum, c->stream)) != hipSuccess)
+        return e;
+    }
+    // U2 = U + dt/2 f2 ; acc += 2 f2
+    if ((e = rhs(c, c->Ua, c->U, c->acc, c->Ub, nullptr, 1.0, 0.0, 0.5 * dt, 2)) != hipSuccess)
+      return e;
+    // U3 = U + dt f3 ; acc += 2 f3
+    if ((e = rhs(c, c->Ub, c->U, c->acc, c->Ua, nullptr, 1.0, 0.0, dt, 2)) != hipSuccess)
+      return e;
+    // U = U + dt/6 (acc + f4)
+    return rhs(c, c->Ua, c->U, c->acc, c->U, nullptr, 1.0, 0.0, dt, 3);
+  }
+  // TVD-RK3: U1 = U + dt f
+  if ((e = rhs(c, c->U, c->U, nullptr, c->Ua, want_norm ? c->fs : nullptr, 1.0, 0.0, dt, 0)) !=
+      hipSuccess)
+    return e;
+  if (want_norm)
+  {
+    if ((e = resid_out_dispatch(c->p, c->q, mesh_args(c), c->fs, nullptr, nullptr, c->partial,
+                                c->stream)) != hipSuccess)
+      return e;
+    if ((e = sum_dispatch(c->partial, c->npartial, c->sum, c->stream)) != hipSuccess)
+      return e;
+  }
+  // U2 = 3/4 U + 1/4 U1 + 1/4 dt f
+  if ((e = rhs(c, c->Ua, c->U, nullptr, c->Ub, nullptr, 0.75, 0.25, 0.25 * dt, 0)) != hipSuccess)
+    return e;
+  // U = 1/3 U + 2/3 U2 + 2/3 dt f
+  return rhs(c, c->Ub, c->U, nullptr, c->U, nullptr, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt, 0);
+}
+
+dgx_status dgx_step_async(dgx_ctx* c, int scheme, double dt, uint64_t nsteps)
+{
+  if (!c || (scheme != DGX_RK4 && scheme != DGX_TVDRK3))
+    return DGX_ERR_INVALID;
+  for (uint64_t s = 0; s < nsteps; ++s)
+    HIPCHK(c, launch_step(c, scheme, dt, false));
+  return DGX_OK;
+}
+
+dgx_status dgx_step(dgx_ctx* c, int scheme, double dt, uint64_t nsteps, double* norms_host)
+{
+  if (!c || (scheme != DGX_RK4 && scheme != DGX_TVDRK3))
+    return DGX_ERR_INVALID;
+  double n = (double)c->nstate;
+  for (uint64_t s = 0; s < nsteps; ++s)
+  {
+    HIPCHK(c, launch_step(c, scheme, dt, norms_host != nullptr));
+    if (norms_host)
+    {
+      double sq = 0.0;
+      HIPCHK(c, hipMemcpyAsync(&sq, c->sum, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      double nrm = sqrt((1.0 / n) * sq);
+      norms_host[s] = nrm;
+      if (std::isnan(nrm))
+        return fail(c, DGX_ERR_NAN, "looks like things went unstable! (step %llu)",
+                    (unsigned long long)s);
+    }
+  }
+  return check_status(c);
+}
+
+}  // extern "C"

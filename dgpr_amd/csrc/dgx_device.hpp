@@ -1,0 +1,261 @@
+// dgx_device.hpp -- device-side tables and pointwise physics for the gfx950
+// DG residual kernels.
+//
+// Pointwise physics restates DGPR's flux.cpp and bound_states.cpp in FP64:
+//   euler_flux      <- analytical_flux (flux.cpp:24-48)
+//   roe_flux        <- roe             (flux.cpp:56-154)
+//   visc            <- A               (flux.cpp:156-202), split into a
+//                      state-dependent setup and a gradient-linear apply
+//   boundary_state  <- bound_states.cpp:24-130
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dgx_launch.hpp"
+
+namespace dgx
+{
+
+__device__ __forceinline__ double rcp(double x)
+{
+  return 1.0 / x;
+}
+
+// Euler flux F[d][r] of conserved state U.
+__device__ __forceinline__ void euler_flux(const double* U, double gamma,
+                                           double F[3][5])
+{
+  double rho = U[0];
+  double ir  = rcp(rho);
+  double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
+  double p = (gamma - 1.0) * (U[4] - 0.5 * rho * (u * u + v * v + w * w));
+  double H = (U[4] + p) * ir;
+  double m[3] = {U[1], U[2], U[3]};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    F[d][0] = m[d];
+    F[d][1] = m[d] * u;
+    F[d][2] = m[d] * v;
+    F[d][3] = m[d] * w;
+    F[d][1 + d] += p;
+    F[d][4] = m[d] * H;
+  }
+}
+
+// Normal Euler flux F(U).n
+__device__ __forceinline__ void euler_flux_n(const double* U, double gamma,
+                                             const double* n, double* Fn)
+{
+  double rho = U[0];
+  double ir  = rcp(rho);
+  double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
+  double p  = (gamma - 1.0) * (U[4] - 0.5 * rho * (u * u + v * v + w * w));
+  double un = u * n[0] + v * n[1] + w * n[2];
+  double mn = rho * un;
+  Fn[0] = mn;
+  Fn[1] = mn * u + p * n[0];
+  Fn[2] = mn * v + p * n[1];
+  Fn[3] = mn * w + p * n[2];
+  Fn[4] = un * (U[4] + p);
+}
+
+// Roe flux with |lambda| entropy fix below 5% of the Roe sound speed.
+__device__ __forceinline__ void roe_flux(const double* Ul, const double* Ur,
+                                         const double* n, double g, double* f)
+{
+  double gm1 = g - 1.0;
+  double irl = rcp(Ul[0]), irr = rcp(Ur[0]);
+  double ul = Ul[1] * irl, vl = Ul[2] * irl, wl = Ul[3] * irl;
+  double ur = Ur[1] * irr, vr = Ur[2] * irr, wr = Ur[3] * irr;
+  double pl = gm1 * (Ul[4] - 0.5 * Ul[0] * (ul * ul + vl * vl + wl * wl));
+  double pr = gm1 * (Ur[4] - 0.5 * Ur[0] * (ur * ur + vr * vr + wr * wr));
+  double Hl = (Ul[4] + pl) * irl;
+  double Hr = (Ur[4] + pr) * irr;
+  double sl = sqrt(Ul[0]), sr = sqrt(Ur[0]);
+  double iden = rcp(sl + sr);
+  double u = (sl * ul + sr * ur) * iden;
+  double v = (sl * vl + sr * vr) * iden;
+  double w = (sl * wl + sr * wr) * iden;
+  double H = (sl * Hl + sr * Hr) * iden;
+  double dr = Ur[0] - Ul[0], dxm = Ur[1] - Ul[1], dym = Ur[2] - Ul[2];
+  double dzm = Ur[3] - Ul[3], drE = Ur[4] - Ul[4];
+  double qsq = u * u + v * v + w * w;
+  double c   = sqrt(gm1 * (H - 0.5 * qsq));
+  double ic  = rcp(c);
+  double un  = u * n[0] + v * n[1] + w * n[2];
+  double l1 = un + c, l2 = un - c, l3 = un;
+  double al1 = fabs(l1), al2 = fabs(l2), al3 = fabs(l3);
+  double eps  = 0.05 * c;
+  double i2e  = rcp(2.0 * eps);
+  if (al1 < eps) al1 = fabs((eps * eps + l1 * l1) * i2e);
+  if (al2 < eps) al2 = fabs((eps * eps + l2 * l2) * i2e);
+  if (al3 < eps) al3 = fabs((eps * eps + l3 * l3) * i2e);
+  double s1 = 0.5 * (al1 + al2);
+  double s2 = 0.5 * (al1 - al2);
+  double G1 = gm1 * (0.5 * qsq * dr - (u * dxm + v * dym + w * dzm) + drE);
+  double G2 = -un * dr + (dxm * n[0] + dym * n[1] + dzm * n[2]);
+  double C1 = (G1 * ic * ic) * (s1 - al3) + (G2 * ic) * s2;
+  double C2 = (G1 * ic) * s2 + (s1 - al3) * G2;
+  double unl = ul * n[0] + vl * n[1] + wl * n[2];
+  double unr = ur * n[0] + vr * n[1] + wr * n[2];
+  double mnl = Ul[0] * unl, mnr = Ur[0] * unr;
+  f[0] = 0.5 * (mnl + mnr) - 0.5 * (al3 * dr + C1);
+  f[1] = 0.5 * ((mnl * ul + pl * n[0]) + (mnr * ur + pr * n[0])) -
+         0.5 * (al3 * dxm + C1 * u + C2 * n[0]);
+  f[2] = 0.5 * ((mnl * vl + pl * n[1]) + (mnr * vr + pr * n[1])) -
+         0.5 * (al3 * dym + C1 * v + C2 * n[1]);
+  f[3] = 0.5 * ((mnl * wl + pl * n[2]) + (mnr * wr + pr * n[2])) -
+         0.5 * (al3 * dzm + C1 * w + C2 * n[2]);
+  f[4] = 0.5 * ((mnl * Hl) + (mnr * Hr)) - 0.5 * (al3 * drE + C1 * H + C2 * un);
+}
+
+// State-dependent part of the Newtonian viscous flux A(s, grad): the
+// velocity and 1/rho.  A is linear in its gradient argument.
+struct ViscState
+{
+  double ir, u, v, w;
+};
+
+__device__ __forceinline__ ViscState visc_setup(const double* s)
+{
+  ViscState vs;
+  vs.ir = rcp(s[0]);
+  vs.u  = s[1] * vs.ir;
+  vs.v  = s[2] * vs.ir;
+  vs.w  = s[3] * vs.ir;
+  return vs;
+}
+
+// Q[d][r] = A(s, G) with conserved-variable gradient G[d][r]; no heat flux,
+// lambda = -2/3 mu.
+__device__ __forceinline__ void visc_apply(const ViscState& vs, const double G[3][5],
+                                           double mu, double Q[3][5])
+{
+  double lam = -(2.0 / 3.0) * mu;
+  double ux = (G[0][1] - G[0][0] * vs.u) * vs.ir;
+  double uy = (G[1][1] - G[1][0] * vs.u) * vs.ir;
+  double uz = (G[2][1] - G[2][0] * vs.u) * vs.ir;
+  double vx = (G[0][2] - G[0][0] * vs.v) * vs.ir;
+  double vy = (G[1][2] - G[1][0] * vs.v) * vs.ir;
+  double vz = (G[2][2] - G[2][0] * vs.v) * vs.ir;
+  double wx = (G[0][3] - G[0][0] * vs.w) * vs.ir;
+  double wy = (G[1][3] - G[1][0] * vs.w) * vs.ir;
+  double wz = (G[2][3] - G[2][0] * vs.w) * vs.ir;
+  double dv  = ux + vy + wz;
+  double txx = lam * dv + 2.0 * mu * ux;
+  double tyy = lam * dv + 2.0 * mu * vy;
+  double tzz = lam * dv + 2.0 * mu * wz;
+  double txy = mu * (vx + uy);
+  double txz = mu * (uz + wx);
+  double tyz = mu * (wy + vz);
+  Q[0][0] = 0.0; Q[0][1] = txx; Q[0][2] = txy; Q[0][3] = txz;
+  Q[0][4] = vs.u * txx + vs.v * txy + vs.w * txz;
+  Q[1][0] = 0.0; Q[1][1] = txy; Q[1][2] = tyy; Q[1][3] = tyz;
+  Q[1][4] = vs.u * txy + vs.v * tyy + vs.w * tyz;
+  Q[2][0] = 0.0; Q[2][1] = txz; Q[2][2] = tyz; Q[2][3] = tzz;
+  Q[2][4] = vs.u * txz + vs.v * tyz + vs.w * tzz;
+}
+
+// Normal component A(s, G).n only.
+__device__ __forceinline__ void visc_apply_n(const ViscState& vs, const double G[3][5],
+                                             double mu, const double* n, double* Qn)
+{
+  double Q[3][5];
+  visc_apply(vs, G, mu, Q);
+#pragma unroll
+  for (int r = 0; r < 5; ++r)
+    Qn[r] = Q[0][r] * n[0] + Q[1][r] * n[1] + Q[2][r] * n[2];
+}
+
+// Boundary state Ub from interior trace UL and outward normal n.
+// Returns false when the subsonic-inflow Mach quadratic has no
+// non-negative root (the reference aborts there on the host).
+__device__ __forceinline__ bool boundary_state(int bt, const double* UL, const double* n,
+                                               const Phys& ph, double* Ub)
+{
+  double gamma = ph.gamma, gm1 = gamma - 1.0;
+  switch (bt)
+  {
+    case -3:  // freestream
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+        Ub[i] = ph.Ufr[i];
+      return true;
+    case -4:  // inviscid wall: remove the normal velocity
+    {
+      double ir = rcp(UL[0]);
+      double up = UL[1] * ir, vp = UL[2] * ir, wp = UL[3] * ir;
+      double vdn = up * n[0] + vp * n[1] + wp * n[2];
+      Ub[0] = UL[0];
+      Ub[1] = UL[0] * (up - vdn * n[0]);
+      Ub[2] = UL[0] * (vp - vdn * n[1]);
+      Ub[3] = UL[0] * (wp - vdn * n[2]);
+      Ub[4] = UL[4];
+      return true;
+    }
+    case -7:  // no-slip wall, energy taken from the interior
+      Ub[0] = UL[0]; Ub[1] = 0.0; Ub[2] = 0.0; Ub[3] = 0.0; Ub[4] = UL[4];
+      return true;
+    case -5:  // subsonic inflow: total T, total P, outgoing Riemann invariant
+    {
+      double R = ph.Rgas, Tt = ph.Tt, Pt = ph.Pt;
+      double ir = rcp(UL[0]);
+      double up = UL[1] * ir, vp = UL[2] * ir, wp = UL[3] * ir;
+      double Vsq = up * up + vp * vp + wp * wp;
+      double pp  = gm1 * (UL[4] - 0.5 * UL[0] * Vsq);
+      double unp = up * n[0] + vp * n[1] + wp * n[2];
+      double cp  = sqrt(gamma * pp * ir);
+      double Jp  = unp + ((2.0 * cp) / gm1);
+      double a   = gamma * R * Tt - 0.5 * gm1 * Jp * Jp;
+      double b   = (4.0 * gamma * R * Tt * -1.0) / gm1;
+      double c   = (4.0 * gamma * R * Tt) / (gm1 * gm1) - Jp * Jp;
+      double disc = sqrt(b * b - 4.0 * a * c);
+      double s0  = (-b + disc) / (2.0 * a);
+      double s1  = (-b - disc) / (2.0 * a);
+      double Mb  = 0.0;
+      bool ok    = true;
+      if (s0 >= 0.0 && s1 >= 0.0)
+        Mb = (s0 < s1) ? s0 : s1;
+      else if (s0 >= 0.0)
+        Mb = s0;
+      else if (s1 >= 0.0)
+        Mb = s1;
+      else
+        ok = false;
+      double Tb   = Tt / (1.0 + 0.5 * gm1 * Mb * Mb);
+      double Pb   = Pt * pow(Tb / Tt, gamma / gm1);
+      double rhob = Pb / (R * Tb);
+      double cb   = sqrt((gamma * Pb) / rhob);
+      double ub = Mb * cb * -n[0], vb = Mb * cb * -n[1], wb = Mb * cb * -n[2];
+      Ub[0] = rhob; Ub[1] = rhob * ub; Ub[2] = rhob * vb; Ub[3] = rhob * wb;
+      Ub[4] = (Pb / gm1) + 0.5 * rhob * (ub * ub + vb * vb + wb * wb);
+      return ok;
+    }
+    case -6:  // subsonic outflow: fixed static pressure, interior entropy
+    {
+      double P   = ph.P;
+      double rp  = UL[0];
+      double ir  = rcp(rp);
+      double up = UL[1] * ir, vp = UL[2] * ir, wp = UL[3] * ir;
+      double Vsq = up * up + vp * vp + wp * wp;
+      double pp  = gm1 * (UL[4] - 0.5 * rp * Vsq);
+      double Sp  = pp / pow(rp, gamma);
+      double rhob = pow(P / Sp, 1.0 / gamma);
+      double cb   = sqrt((gamma * P) / rhob);
+      double unp  = up * n[0] + vp * n[1] + wp * n[2];
+      double cp   = sqrt(gamma * pp * ir);
+      double Jp   = unp + ((2.0 * cp) / gm1);
+      double unb  = Jp - ((2.0 * cb) / gm1);
+      double ub = up - unp * n[0] + unb * n[0];
+      double vb = vp - unp * n[1] + unb * n[1];
+      double wb = wp - unp * n[2] + unb * n[2];
+      Ub[0] = rhob; Ub[1] = rhob * ub; Ub[2] = rhob * vb; Ub[3] = rhob * wb;
+      Ub[4] = (P / gm1) + 0.5 * rhob * (ub * ub + vb * vb + wb * wb);
+      return true;
+    }
+  }
+  return false;
+}
+
+}  // namespace dgx

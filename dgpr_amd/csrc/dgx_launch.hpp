@@ -1,0 +1,86 @@
+// dgx_launch.hpp -- plain structs and launch entry points shared by the host
+// (C ABI) and device translation units of libdgx.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+namespace dgx
+{
+
+constexpr int MAXN = 4;  // (p+1) <= 4 and (q+1) <= 4
+
+// 1D tables of solution order P on the (P+1)-point Gauss rule of [0,1].
+//   g, w       Gauss points / weights
+//   B[i][m]    equispaced Lagrange l_m(g_i)          (reference basis)
+//   Binv       inverse of B
+//   Dg[i][m]   d/dx of the Gauss-node Lagrange l^g_m at g_i
+//   L[e][m]    l^g_m(e), e = 0 (x=0) or 1 (x=1)
+//   Ld[e][m]   d/dx l^g_m at e
+//   Blo, Binvlo  low parts of B and Binv as double-double pairs (hi + lo),
+//              used by the state basis conversions
+struct PTab
+{
+  double g[MAXN], w[MAXN];
+  double B[MAXN][MAXN], Binv[MAXN][MAXN], Dg[MAXN][MAXN];
+  double L[2][MAXN], Ld[2][MAXN];
+  double Blo[MAXN][MAXN], Binvlo[MAXN][MAXN];
+};
+
+// Geometry-order Q equispaced Lagrange values/derivatives at the Gauss points
+// of order P and at the two ends of [0,1].
+struct QTab
+{
+  double Bq[MAXN][MAXN], Bqd[MAXN][MAXN];
+  double Be[2][MAXN], Bde[2][MAXN];
+};
+
+// Physical constants of one evaluation (parameters + derived boundary data).
+struct Phys
+{
+  double gamma, mu, eta, Rgas, P, Pt, Tt;
+  double Ufr[5];
+  double source[5];
+};
+
+// Structured mesh: nodes [e][(q+1)^3][3], element e = (k*ny + j)*nx + i,
+// boundary codes per side -x +x -y +y -z +z.
+struct MeshArgs
+{
+  const double* nodes;
+  int nx, ny, nz;
+  int nelem;
+  int bounds[6];
+};
+
+// Epilogue of one right-hand-side evaluation:
+//   Vout = ca*Vbase + cb*Vin + cf*fterm
+// with fterm = f for accmode 0..2 and (acc + f)/6 for accmode 3;
+// accmode 1 stores acc = f, accmode 2 updates acc += 2 f.  fout (optional)
+// receives f.  All states are Gauss-nodal [e][5][(p+1)^3].
+struct StageArgs
+{
+  const double* Vin;
+  const double* Vbase;
+  double* acc;
+  double* Vout;
+  double* fout;
+  double ca, cb, cf;
+  int accmode;
+};
+
+hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
+                        int* status, hipStream_t st);
+hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg, double* f,
+                              double* R, double* partial, hipStream_t st);
+hipError_t basis_dispatch(int p, const double* in, double* out, int nelem, int mode,
+                          hipStream_t st);
+hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st);
+hipError_t upload_tables(const PTab* pt, const QTab* qt);
+int resid_out_blocks(int p, int nelem);
+
+namespace host
+{
+void build_tables(PTab* pt, QTab* qt);
+}
+
+}  // namespace dgx

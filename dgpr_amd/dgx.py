@@ -1,0 +1,305 @@
+"""Python binding of libdgx (the gfx950 DG residual / RK library) over its C ABI.
+
+This mirrors the reference's host-side GPU API (solver/cudg.cu,
+solve_cuda.cu, time_cuda.cu) on numpy arrays:
+
+=====================================  =====================================
+reference                              here
+=====================================  =====================================
+``parameters`` / make_reference_state ``Params`` / ``make_reference_state``
+``core_geometry`` nodes + bounds       ``Mesh``
+``bump_shared`` node generation         ``bump_nodes``
+``cuda_device_geometry`` + workspace    ``Solver(mesh, params)``
+shuffle_state + cudaMemcpy              ``Solver.set_state`` / ``get_state``
+``cuda_residual``                       ``Solver.residual``
+``RK4_cuda`` / ``tvdRK3_cuda``          ``Solver.step``
+``write_state`` / ``read_state``        ``write_dg`` / ``read_dg``
+=====================================  =====================================
+
+The library must exist (built by ``dgpr_amd/build.py``); there is no CPU
+fallback: every compute call goes to the HIP kernels and raises on failure.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdgx.so")
+
+D = C.c_double
+PD = C.POINTER(C.c_double)
+U64 = C.c_uint64
+
+# btype codes (boundaries.cpp:26-35)
+INTERIOR, PERIODIC, FREESTREAM, INVISCID_WALL = -1, -2, -3, -4
+SUBSONIC_INFLOW, SUBSONIC_OUTFLOW, NO_SLIP_WALL = -5, -6, -7
+BTYPES = {"periodic": PERIODIC, "freestream": FREESTREAM, "inviscid_wall": INVISCID_WALL,
+          "subsonic_inflow": SUBSONIC_INFLOW, "subsonic_outflow": SUBSONIC_OUTFLOW,
+          "no_slip_wall": NO_SLIP_WALL}
+# time_scheme (time.cpp:23-27)
+RK4, TVDRK3 = 0, 1
+SCHEMES = {"RK4": RK4, "tvdRK3": TVDRK3}
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "HIP error",
+          4: "NaN residual norm", 5: "I/O error", 6: "no HIP device"}
+
+
+class DgxError(RuntimeError):
+    def __init__(self, status, msg=""):
+        super().__init__("dgx status %d (%s)%s" % (status, STATUS.get(status, "?"),
+                                                   (": " + msg) if msg else ""))
+        self.status = status
+
+
+class Params(C.Structure):
+    """``parameters`` (solver_params.cpp:22-36)."""
+    _fields_ = [("gamma", D), ("R", D), ("mu", D), ("c", D), ("T", D), ("P", D),
+                ("Ufr", D * 5), ("source", D * 5), ("eta", D)]
+
+    def copy(self):
+        p = Params()
+        C.memmove(C.byref(p), C.byref(self), C.sizeof(Params))
+        return p
+
+
+class _CMesh(C.Structure):
+    _fields_ = [("p", U64), ("q", U64), ("nx", U64), ("ny", U64), ("nz", U64),
+                ("bounds", C.c_int64 * 6), ("nodes", PD)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libdgx.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DgxError(2, "libdgx.so not built: run python -m dgpr_amd.build")
+        L = C.CDLL(LIB_PATH)
+        st = C.c_int
+        P_ctx = C.c_void_p
+        L.dgx_create.argtypes = [C.POINTER(_CMesh), C.POINTER(Params), C.POINTER(P_ctx)]
+        L.dgx_destroy.argtypes = [P_ctx]
+        L.dgx_set_params.argtypes = [P_ctx, C.POINTER(Params)]
+        L.dgx_set_state.argtypes = [P_ctx, PD]
+        L.dgx_get_state.argtypes = [P_ctx, PD]
+        L.dgx_set_state_device.argtypes = [P_ctx, C.c_void_p]
+        L.dgx_get_state_device.argtypes = [P_ctx, C.c_void_p]
+        L.dgx_residual.argtypes = [P_ctx, PD, PD]
+        L.dgx_residual_device.argtypes = [P_ctx, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.dgx_step.argtypes = [P_ctx, C.c_int, D, U64, PD]
+        L.dgx_step_async.argtypes = [P_ctx, C.c_int, D, U64]
+        L.dgx_sync.argtypes = [P_ctx]
+        L.dgx_get_stream.argtypes = [P_ctx, C.POINTER(C.c_void_p)]
+        L.dgx_get_sizes.argtypes = [P_ctx, C.POINTER(U64)]
+        L.dgx_last_error.argtypes = [P_ctx]
+        L.dgx_last_error.restype = C.c_char_p
+        L.dgx_make_reference_state.argtypes = [D, D, D, D, D, D, D, D, D, PD, C.POINTER(Params)]
+        L.dgx_bump_nodes.argtypes = [U64, U64, U64, U64, D, D, D, D, D, D, D, PD]
+        L.dgx_box_nodes.argtypes = [U64, U64, U64, U64, D, D, D, D, D, D, D, U64, PD]
+        L.dgx_tgv_state.argtypes = [C.POINTER(_CMesh), D, D, D, D, D, U64, PD]
+        L.dgx_channel_state.argtypes = [C.POINTER(_CMesh), D, D, D, D, D, C.c_int, D, D, U64, PD]
+        L.dgx_write_dg.argtypes = [C.c_char_p, U64, C.POINTER(_CMesh), PD]
+        L.dgx_read_dg_header.argtypes = [C.c_char_p, C.POINTER(U64), C.POINTER(_CMesh)]
+        L.dgx_read_dg.argtypes = [C.c_char_p, PD, PD]
+        for nm in ("dgx_create", "dgx_destroy", "dgx_set_params", "dgx_set_state",
+                   "dgx_get_state", "dgx_set_state_device", "dgx_get_state_device",
+                   "dgx_residual", "dgx_residual_device", "dgx_step", "dgx_step_async",
+                   "dgx_sync", "dgx_get_stream", "dgx_get_sizes", "dgx_make_reference_state",
+                   "dgx_bump_nodes", "dgx_box_nodes", "dgx_tgv_state", "dgx_channel_state",
+                   "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg"):
+            getattr(L, nm).restype = st
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(PD)
+
+
+def _chk(status, ctx=None):
+    if status != 0:
+        msg = lib().dgx_last_error(ctx)
+        raise DgxError(status, msg.decode() if msg else "")
+
+
+def make_reference_state(gamma=1.4, R=287.0, mu=1e-3, c=1.0, rho=1.0, u=0.1, v=0.0, w=0.0,
+                         eta=6.0, source=None):
+    """``make_reference_state`` (solver_params.cpp:40-70)."""
+    prm = Params()
+    src = None
+    if source is not None:
+        src = np.ascontiguousarray(source, dtype=np.float64)
+    _chk(lib().dgx_make_reference_state(gamma, R, mu, c, rho, u, v, w, eta,
+                                        _p(src) if src is not None else None, C.byref(prm)))
+    return prm
+
+
+class Mesh:
+    """Persistent part of ``core_geometry``: orders, element counts, boundary
+    codes (-x +x -y +y -z +z) and element nodes [e][(q+1)^3][3]."""
+
+    def __init__(self, p, q, nx, ny, nz, bounds, nodes):
+        self.p, self.q, self.nx, self.ny, self.nz = int(p), int(q), int(nx), int(ny), int(nz)
+        self.bounds = np.asarray(bounds, dtype=np.int64).copy()
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1)
+        nq = self.q + 1
+        if self.nodes.size != self.nelem * nq ** 3 * 3:
+            raise ValueError("nodes has %d entries, expected %d"
+                             % (self.nodes.size, self.nelem * nq ** 3 * 3))
+
+    @property
+    def nelem(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def nbf(self):
+        return (self.p + 1) ** 3
+
+    @property
+    def state_size(self):
+        return self.nelem * 5 * self.nbf
+
+    def cstruct(self):
+        m = _CMesh()
+        m.p, m.q, m.nx, m.ny, m.nz = self.p, self.q, self.nx, self.ny, self.nz
+        for i in range(6):
+            m.bounds[i] = int(self.bounds[i])
+        m.nodes = _p(self.nodes)
+        return m
+
+
+def bump_nodes(q, nx, ny, nz, lx, ly, lz, a=0.0, bmp_height=0.0, bmp_center=1.0,
+               bmp_variance=0.05):
+    """Nodes of ``bump_shared`` (geometry_common.cpp:494-576)."""
+    out = np.zeros(nx * ny * nz * (q + 1) ** 3 * 3)
+    _chk(lib().dgx_bump_nodes(q, nx, ny, nz, lx, ly, lz, a, bmp_height, bmp_center,
+                              bmp_variance, _p(out)))
+    return out
+
+
+def box_nodes(q, nx, ny, nz, x0, y0, z0, lx, ly, lz, amp=0.0, seed=0):
+    """Box nodes with an optional smooth, x/z-periodic displacement that
+    vanishes on the y planes (relative amplitude ``amp`` of the element size)."""
+    out = np.zeros(nx * ny * nz * (q + 1) ** 3 * 3)
+    _chk(lib().dgx_box_nodes(q, nx, ny, nz, x0, y0, z0, lx, ly, lz, amp, seed, _p(out)))
+    return out
+
+
+def tgv_state(mesh, gamma=1.4, rho0=1.0, V0=1.0, P0=None, noise=1e-3, seed=0):
+    """Taylor-Green-type state at the Lagrange nodes plus seeded velocity noise."""
+    if P0 is None:
+        raise ValueError("P0 required")
+    out = np.zeros(mesh.state_size)
+    cm = mesh.cstruct()
+    _chk(lib().dgx_tgv_state(C.byref(cm), gamma, rho0, V0, P0, noise, seed, _p(out)))
+    return out
+
+
+def channel_state(mesh, gamma, rho0, Uc, P0, frac, kmax, lx, lz, seed):
+    """Poiseuille profile plus divergence-free Fourier noise."""
+    out = np.zeros(mesh.state_size)
+    cm = mesh.cstruct()
+    _chk(lib().dgx_channel_state(C.byref(cm), gamma, rho0, Uc, P0, frac, kmax, lx, lz, seed,
+                                 _p(out)))
+    return out
+
+
+def write_dg(path, tstep, mesh, U):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    cm = mesh.cstruct()
+    _chk(lib().dgx_write_dg(path.encode(), tstep, C.byref(cm), _p(U)))
+
+
+def read_dg(path):
+    """Returns (tstep, Mesh, U) of a .dg file's "state" output."""
+    ts = U64()
+    cm = _CMesh()
+    _chk(lib().dgx_read_dg_header(path.encode(), C.byref(ts), C.byref(cm)))
+    nq = cm.q + 1
+    ne = cm.nx * cm.ny * cm.nz
+    nodes = np.zeros(ne * nq ** 3 * 3)
+    U = np.zeros(ne * 5 * (cm.p + 1) ** 3)
+    _chk(lib().dgx_read_dg(path.encode(), _p(nodes), _p(U)))
+    mesh = Mesh(cm.p, cm.q, cm.nx, cm.ny, cm.nz, list(cm.bounds), nodes)
+    return int(ts.value), mesh, U
+
+
+class Solver:
+    """Device context: geometry, state and stage buffers on the current HIP
+    device (``cuda_device_geometry`` + ``cuda_residual_workspace``)."""
+
+    def __init__(self, mesh, params):
+        self.mesh = mesh
+        self.params = params.copy()
+        self._cm = mesh.cstruct()
+        ctx = C.c_void_p()
+        st = lib().dgx_create(C.byref(self._cm), C.byref(self.params), C.byref(ctx))
+        _chk(st, None)
+        self.ctx = ctx
+        s = (U64 * 4)()
+        _chk(lib().dgx_get_sizes(self.ctx, s), self.ctx)
+        self.nelem, self.nbf, self.nstate, self.device_bytes = [int(v) for v in s]
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().dgx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, params):
+        self.params = params.copy()
+        _chk(lib().dgx_set_params(self.ctx, C.byref(self.params)), self.ctx)
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1)
+        if U.size != self.nstate:
+            raise ValueError("state has %d entries, expected %d" % (U.size, self.nstate))
+        _chk(lib().dgx_set_state(self.ctx, _p(U)), self.ctx)
+
+    def get_state(self):
+        U = np.zeros(self.nstate)
+        _chk(lib().dgx_get_state(self.ctx, _p(U)), self.ctx)
+        return U
+
+    def residual(self):
+        """(R, f) of the current state in simstate layout (solve.cpp:150-807)."""
+        R = np.zeros(self.nstate)
+        f = np.zeros(self.nstate)
+        _chk(lib().dgx_residual(self.ctx, _p(R), _p(f)), self.ctx)
+        return R, f
+
+    def step(self, dt, scheme="RK4", nsteps=1, norms=True):
+        """Advance nsteps; returns per-step RMS(R) norms (or None)."""
+        sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        if norms:
+            out = np.zeros(nsteps)
+            _chk(lib().dgx_step(self.ctx, sc, dt, nsteps, _p(out)), self.ctx)
+            return out
+        _chk(lib().dgx_step(self.ctx, sc, dt, nsteps, None), self.ctx)
+        return None
+
+    def step_async(self, dt, scheme="RK4", nsteps=1):
+        sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        _chk(lib().dgx_step_async(self.ctx, sc, dt, nsteps), self.ctx)
+
+    def sync(self):
+        _chk(lib().dgx_sync(self.ctx), self.ctx)
+
+    def stream(self):
+        s = C.c_void_p()
+        _chk(lib().dgx_get_stream(self.ctx, C.byref(s)), self.ctx)
+        return s.value

@@ -1,0 +1,229 @@
+/*
+ * dgx.h -- C ABI of libdgx, the MI355X (gfx950) DG residual and explicit
+ * time-step library.
+ *
+ * The reference (DGPR) has no plugin API: its GPU path is a set of free
+ * functions and POD structs #included into solver/cudg.cu.  Each entry point
+ * below names the reference interface it replaces (file:line in DGPR).
+ * All functions return a dgx_status; none aborts the process.  Every buffer
+ * is a plain pointer; sizes are implied by the mesh (nelem*5*(p+1)^3 doubles
+ * for a state) or passed explicitly.
+ *
+ * State layout at this boundary is the reference's simstate layout
+ * U[e][r][bf] (geometry_common.cpp:467-470): conserved variable r of
+ * equispaced-Lagrange coefficient bf on element e, elements ordered
+ * e = nx*ny*k + nx*j + i, coefficients ordered z-y-x.
+ */
+#ifndef DGX_H
+#define DGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dgx_status
+{
+  DGX_OK              = 0,
+  DGX_ERR_INVALID     = 1, /* bad argument (null pointer, size, order)     */
+  DGX_ERR_UNSUPPORTED = 2, /* valid input outside what the kernels cover   */
+  DGX_ERR_HIP         = 3, /* a HIP runtime call failed                    */
+  DGX_ERR_NAN         = 4, /* residual norm is NaN ("went unstable")       */
+  DGX_ERR_IO          = 5, /* .dg file could not be read or written        */
+  DGX_ERR_NODEVICE    = 6  /* no HIP device visible                        */
+} dgx_status;
+
+/* btype (boundaries.cpp:26-35): negative codes stored in bounds[6] */
+enum
+{
+  DGX_BT_INTERIOR         = -1,
+  DGX_BT_PERIODIC         = -2,
+  DGX_BT_FREESTREAM       = -3,
+  DGX_BT_INVISCID_WALL    = -4,
+  DGX_BT_SUBSONIC_INFLOW  = -5,
+  DGX_BT_SUBSONIC_OUTFLOW = -6,
+  DGX_BT_NO_SLIP_WALL     = -7
+};
+
+/* time_scheme (time.cpp:23-27) */
+enum
+{
+  DGX_RK4    = 0,
+  DGX_TVDRK3 = 1
+};
+
+/* Arithmetic of the interpolation/projection contractions. */
+enum
+{
+  DGX_PREC_FP64 = 0 /* plain FP64 form */
+};
+
+/* parameters (solver_params.cpp:22-36), passed by value to the kernels
+ * exactly as the reference passes it to cuda_residual. */
+typedef struct dgx_params
+{
+  double gamma;     /* ratio of specific heats        */
+  double R;         /* specific gas constant           */
+  double mu;        /* dynamic viscosity               */
+  double c;         /* reference speed of sound        */
+  double T;         /* reference static temperature    */
+  double P;         /* reference static pressure       */
+  double Ufr[5];    /* free-stream conserved state     */
+  double source[5]; /* constant source term            */
+  double eta;       /* BR2 stabilisation parameter     */
+} dgx_params;
+
+/* The persistent part of core_geometry (geometry_common.cpp:25-83), i.e.
+ * what a .dg file stores (io.cpp:33-89): orders, element counts, boundary
+ * codes per domain side (-x +x -y +y -z +z) and element nodes
+ * [e][(q+1)^3][3].  Face lists are implied by the structured layout exactly
+ * as shared_geometry::precompute_face_geometry derives them
+ * (geometry_shared.cpp:156-267). */
+typedef struct dgx_mesh
+{
+  uint64_t p, q;
+  uint64_t nx, ny, nz;
+  int64_t bounds[6];
+  const double* nodes;
+} dgx_mesh;
+
+typedef struct dgx_ctx dgx_ctx;
+
+/* ---- context: device geometry + residual workspace ----------------------
+ * Replaces cuda_device_geometry::cuda_device_geometry(shared_geometry&)
+ * (solve_cuda.cu:171) together with cuda_residual_workspace(...)
+ * (solve_cuda.cu:303): uploads the mesh, allocates the state and the stage
+ * buffers of both RK schemes on the current HIP device and creates one
+ * stream.  Supported: 1 <= p <= 3, 1 <= q <= 3. */
+dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params,
+                      dgx_ctx** out);
+
+/* Replaces free_cuda_device_geometry (solve_cuda.cu:267) and
+ * free_cuda_residual_workspace (solve_cuda.cu:323). */
+dgx_status dgx_destroy(dgx_ctx* ctx);
+
+/* Replaces the parameters argument of cuda_residual / RK4_cuda
+ * (solve_cuda.cu:1235, time_cuda.cu:152): the reference state can change
+ * between calls (e.g. viscosity sweeps). */
+dgx_status dgx_set_params(dgx_ctx* ctx, const dgx_params* params);
+
+/* Replaces shuffle_state + cudaMemcpy H2D (cudg.cu:97-106): U_host is a
+ * simstate [e][5][nbf]. */
+dgx_status dgx_set_state(dgx_ctx* ctx, const double* U_host);
+
+/* Replaces cudaMemcpy D2H + unshuffle_state (cudg.cu:131-136). */
+dgx_status dgx_get_state(dgx_ctx* ctx, double* U_host);
+
+/* Device-pointer forms of the two transfers above: U_dev is a device
+ * buffer in simstate layout. */
+dgx_status dgx_set_state_device(dgx_ctx* ctx, const double* U_dev);
+dgx_status dgx_get_state_device(dgx_ctx* ctx, double* U_dev);
+
+/* Replaces cuda_residual(state, geom, params, wsp, residual, f)
+ * (solve_cuda.cu:1235-1282) with the CPU semantics of residual()
+ * (solve.cpp:150-807): R = volume - source + faces, f = -Minv R.  All three
+ * are device buffers in simstate layout; R_dev may be NULL.  Runs on the
+ * context's stream and returns after completion. */
+dgx_status dgx_residual_device(dgx_ctx* ctx, const double* U_dev,
+                               double* R_dev, double* f_dev);
+
+/* Host-buffer convenience form: evaluates the residual of the context's
+ * current state; R_host may be NULL. */
+dgx_status dgx_residual(dgx_ctx* ctx, double* R_host, double* f_host);
+
+/* Replaces the step function pointer of cudg.cu:75-91, i.e. RK4_cuda
+ * (time_cuda.cu:152) and tvdRK3_cuda (time_cuda.cu:126), with the CPU
+ * stage algebra of RK4_distributed / tvdRK3_distributed
+ * (time_distributed.cpp:46-102).  Advances the context's state by nsteps
+ * steps of size dt.  If norms_host is not NULL it receives, per step, the
+ * RMS of R at the first stage (residual_norm, time_distributed.cpp:23-44)
+ * and a NaN norm stops stepping with DGX_ERR_NAN. */
+dgx_status dgx_step(dgx_ctx* ctx, int scheme, double dt, uint64_t nsteps,
+                    double* norms_host);
+
+/* Enqueue-only form of dgx_step without norms: launches the stages of
+ * nsteps steps on the context's stream and returns without waiting. */
+dgx_status dgx_step_async(dgx_ctx* ctx, int scheme, double dt,
+                          uint64_t nsteps);
+
+/* Waits for all work on the context's stream. */
+dgx_status dgx_sync(dgx_ctx* ctx);
+
+/* The context's hipStream_t (as void*), for callers that time or order
+ * work around the library with HIP events. */
+dgx_status dgx_get_stream(dgx_ctx* ctx, void** stream);
+
+/* Sizes: out[0]=nelem, out[1]=nbf=(p+1)^3, out[2]=state length
+ * nelem*5*nbf, out[3]=device bytes held by the context. */
+dgx_status dgx_get_sizes(const dgx_ctx* ctx, uint64_t out[4]);
+
+/* Human-readable text of the last error recorded on ctx (or of the last
+ * failed dgx_create when ctx is NULL). */
+const char* dgx_last_error(const dgx_ctx* ctx);
+
+/* ---- host-side helpers (no device work) --------------------------------- */
+
+/* make_reference_state (solver_params.cpp:40-70): P = c^2 rho/gamma,
+ * T = P/(rho R), Ufr from (rho, u, v, w, P).  source may be NULL. */
+dgx_status dgx_make_reference_state(double gamma, double R, double mu,
+                                    double c, double rho, double u, double v,
+                                    double w, double eta,
+                                    const double* source, dgx_params* out);
+
+/* generate_bump_nodes as called by bump_shared (geometry_common.cpp:494-576,
+ * geometry_shared.cpp:275-292): nodes [e][(q+1)^3][3] of an nx*ny*nz
+ * element box of size lx*ly*lz with wall-normal tanh stretching a and a
+ * Gaussian bump on the lower y wall.  nodes_out holds
+ * nx*ny*nz*(q+1)^3*3 doubles. */
+dgx_status dgx_bump_nodes(uint64_t q, uint64_t nx, uint64_t ny, uint64_t nz,
+                          double lx, double ly, double lz, double a,
+                          double bmp_height, double bmp_center,
+                          double bmp_variance, double* nodes_out);
+
+/* Box nodes on [x0,x0+lx]x[y0,y0+ly]x[z0,z0+lz] with each node displaced by
+ * a smooth field of relative amplitude amp (fraction of the local element
+ * size) that is periodic in x and z and vanishes on the y=y0 and y=y0+ly
+ * planes.  amp = 0 gives an affine box.  seed selects the field's phases. */
+dgx_status dgx_box_nodes(uint64_t q, uint64_t nx, uint64_t ny, uint64_t nz,
+                         double x0, double y0, double z0, double lx,
+                         double ly, double lz, double amp, uint64_t seed,
+                         double* nodes_out);
+
+/* Taylor-Green-type initial state sampled at the Lagrange nodes (as
+ * solver_init.cpp:43-76 samples at nodes):
+ *   u = V0 sin x cos y cos z, v = -V0 cos x sin y cos z, w = 0,
+ *   p = P0 + (rho0 V0^2/16)(cos 2x + cos 2y)(cos 2z + 2),
+ *   rho = rho0 p/P0 (isothermal), plus uniform noise in [-noise, noise] on
+ *   each velocity component from a splitmix64 stream seeded by seed.
+ * U_out is a simstate [e][5][nbf]. */
+dgx_status dgx_tgv_state(const dgx_mesh* mesh, double gamma, double rho0,
+                         double V0, double P0, double noise, uint64_t seed,
+                         double* U_out);
+
+/* Channel initial state: laminar Poiseuille u = Uc(1 - y^2) on y in
+ * [-1, 1], rho = rho0, p = P0, plus divergence-free noise built as the
+ * curl of a random vector potential with Fourier modes |m| <= kmax
+ * (periodic in x and z, damped to zero at the walls), scaled to an RMS of
+ * frac*Uc over the nodes. */
+dgx_status dgx_channel_state(const dgx_mesh* mesh, double gamma, double rho0,
+                             double Uc, double P0, double frac, int kmax,
+                             double lx, double lz, uint64_t seed,
+                             double* U_out);
+
+/* .dg checkpoint file (io.cpp:33-159): write_state with one output named
+ * "state", and read_state of that output.  dgx_read_dg_header fills
+ * tstep and a mesh header (nodes NULL); dgx_read_dg then reads nodes and
+ * state into caller buffers sized from that header. */
+dgx_status dgx_write_dg(const char* path, uint64_t tstep, const dgx_mesh* mesh,
+                        const double* U);
+dgx_status dgx_read_dg_header(const char* path, uint64_t* tstep,
+                              dgx_mesh* mesh);
+dgx_status dgx_read_dg(const char* path, double* nodes_out, double* U_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGX_H */

@@ -1,0 +1,22 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    """The oracle (CPU restatement of the reference residual)."""
+    from oracle import build, oracle
+    if not os.path.exists(oracle.LIB_PATH):
+        build.build()
+    oracle.lib()
+    return oracle
