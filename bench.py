@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the gfx950 DG residual + explicit RK path.
+
+Workload (default): the 48x24x24 channel, p = 3 (64 DOF/element), q = 1 with
+smoothly displaced interior nodes (non-affine elements), periodic in x and z,
+no-slip walls at y = +-1, Mach 0.2, Re 3000, TVD-RK3 at CFL 0.3; synthetic
+seeded Poiseuille + divergence-free noise initial state.  One "step" is one
+TVD-RK3 time step (three fused residual + stage kernels) over the whole mesh,
+with the state resident in HBM.
+
+Prints one JSON line:
+  value      = DOF * stages * steps / time / 1e9            [GDOF-stage/s]
+  roofline   = algorithmic HBM bytes per stage-kernel launch / average launch
+               time (HIP events on the library's stream) vs 8 TB/s
+  cpu_baseline = the oracle (CPU restatement of the reference's dense
+               residual + RK) on a bounded sample, timed on this host's cores
+
+Only one GPU is used: --gpus N > 1 exits with a message.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from dgpr_amd import cases, dgx  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def hip():
+    lib = C.CDLL("libamdhip64.so.7")
+    lib.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    lib.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    lib.hipEventSynchronize.argtypes = [C.c_void_p]
+    lib.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    lib.hipEventDestroy.argtypes = [C.c_void_p]
+    lib.hipDeviceSynchronize.argtypes = []
+    lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    lib.hipFree.argtypes = [C.c_void_p]
+    return lib
+
+
+def alg_bytes_per_dof_stage(case):
+    """Compulsory HBM bytes per DOF per stage (FP64, 5 variables).
+
+    Each full-state array costs 40 B/DOF.  TVD-RK3 fused stages read/write
+    (U->U1), (U1,U->U2), (U2,U->U): 8 arrays over 3 stages.  Classic RK4 with
+    an accumulator: (U->U1,acc), (U1,U,acc->U2,acc), (U2,U,acc->U3,acc),
+    (U3,U,acc->U): 17 arrays over 4 stages.  Curved elements add their nodes
+    (24 doubles per q=1 element) once per stage."""
+    arrays = 8.0 / 3.0 if case.scheme == "tvdRK3" else 17.0 / 4.0
+    nq = case.mesh.q + 1
+    geom = 8.0 * 3 * nq ** 3 / case.mesh.nbf
+    return 40.0 * arrays + geom
+
+
+def cpu_baseline(seconds_hint=20.0):
+    """Oracle timing on a bounded sample of the same workload: TVD-RK3 steps
+    of a 12x24x6 block of the p=3 curved channel (same per-element work)."""
+    from oracle import oracle as orc
+    c = cases.channel_case(nx=12, ny=24, nz=6, p=3, seed=1)
+    m = c.mesh
+    g = orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
+    prm = orc.params_from(c.params)
+    U = c.U0.copy()
+    t0 = time.perf_counter()
+    nsteps = 0
+    while True:
+        U, _ = g.step(prm, U, c.dt, c.scheme, 1)
+        nsteps += 1
+        el = time.perf_counter() - t0
+        if el > seconds_hint or nsteps >= 8:
+            break
+    val = c.dof * c.stages * nsteps / el / 1e9
+    return {"value": val, "unit": "GDOF-stage/s", "cores": 1, "kind": "port",
+            "sample": "%d TVD-RK3 step(s) of a 12x24x6 p=3 curved channel block "
+                      "(1728 elements, 110592 DOF), oracle on 1 host core, %.1f s"
+                      % (nsteps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="config1", choices=["config0", "config1", "config2"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one copy kernel of this size (traffic calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build runs the path on a single "
+              "MI355X; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    case = {"config0": cases.config0, "config1": cases.config1, "config2": cases.config2}[a.workload]()
+    h = hip()
+    s = dgx.Solver(case.mesh, case.params)
+    s.set_state(case.U0)
+    stream = C.c_void_p(s.stream())
+    s.step_async(case.dt, case.scheme, a.warmup)
+    s.sync()
+    h.hipDeviceSynchronize()
+
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    h.hipEventCreate(C.byref(ev0))
+    h.hipEventCreate(C.byref(ev1))
+    t0 = time.perf_counter()
+    h.hipEventRecord(ev0, stream)
+    s.step_async(case.dt, case.scheme, a.steps)
+    h.hipEventRecord(ev1, stream)
+    h.hipEventSynchronize(ev1)
+    s.sync()
+    h.hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    ms = C.c_float()
+    h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    gpu_s = ms.value * 1e-3
+
+    # the timed region holds exactly steps*stages stage-kernel launches
+    launches = a.steps * case.stages
+    avg_launch_s = gpu_s / launches
+    bpd = alg_bytes_per_dof_stage(case)
+    bytes_per_launch = bpd * case.dof
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    value = case.dof * case.stages * a.steps / gpu_s / 1e9
+
+    if a.calibrate_copy_mib:
+        nb = a.calibrate_copy_mib << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        h.hipMalloc(C.byref(src), nb)
+        h.hipMalloc(C.byref(dst), nb)
+        dgx.lib().dgx_device_copy(s.ctx, dst, src, nb)
+        s.sync()
+        h.hipFree(src)
+        h.hipFree(dst)
+
+    U = s.get_state()
+    finite = bool(dgx.np.isfinite(U).all())
+    s.close()
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % a.workload)
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+
+    out = {
+        "metric": "GDOF-stage-updates/s, mixed vs FP64 GPU at stated error; adjoint/forward cost",
+        "value": value,
+        "unit": "GDOF-stage/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * gpu_s / a.steps,
+        "wall_ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded mesh and initial state)",
+        "state_finite": finite,
+        "config": {
+            "workload": case.name,
+            "description": {
+                "config0": "8^3 periodic Taylor-Green p=2, RK4",
+                "config1": "48x24x24 curved channel p=3 q=1, no-slip walls, TVD-RK3 CFL 0.3",
+                "config2": "64^3 periodic Taylor-Green p=3, RK4",
+            }[a.workload],
+            "nelem": case.mesh.nelem, "p": case.mesh.p, "q": case.mesh.q,
+            "dof": case.dof, "scheme": case.scheme, "dt": case.dt,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "dgx_rhs_stage_kernel<%d,%d>" % (case.mesh.p, case.mesh.q),
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "alg_bytes_per_dof_stage": bpd,
+            "avg_launch_us": avg_launch_s * 1e6,
+        },
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -30,15 +30,19 @@ def _run(cmd, verbose):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = False, jobs: int = 4, out: str = OUT, defines=()) -> str:
+    """Compile every source and link ``out``; ``defines`` are extra -D flags
+    (kernel tuning parameters such as DGX_STAGE_WAVES_PER_EU=2)."""
+    build_dir = BUILD if out == OUT else out + ".objs"
+    os.makedirs(build_dir, exist_ok=True)
     objs = []
     procs = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
-        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmd = ["hipcc", "-x", "hip"] + COMMON + DEVICE + ["-c", path, "-o", obj]
+        cmd = (["hipcc", "-x", "hip"] + COMMON + DEVICE + ["-D" + d for d in defines]
+               + ["-c", path, "-o", obj])
         if src.endswith(".hip") and os.environ.get("DGX_RESOURCE_USAGE"):
             cmd += ["-Rpass-analysis=kernel-resource-usage"]
         if verbose:
@@ -50,10 +54,10 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     for p in procs:
         if p.wait() != 0:
             raise RuntimeError("compile failed")
-    _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", OUT + ".tmp"] + objs,
+    _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", out + ".tmp"] + objs,
          verbose)
-    os.replace(OUT + ".tmp", OUT)
-    return OUT
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -38,6 +38,7 @@ struct dgx_ctx
   double* fs;       // f_g scratch
   double* partial;  // per-block sums of R^2
   double* sum;      // reduced sum
+  double* lam;      // BR2 lifting coefficients [e][6][(p+1)^2]
   int* status;      // kernel-side error flags
   int npartial;
   size_t bytes;
@@ -228,14 +229,19 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
     return bail("hipMalloc(partial)", e);
   if ((e = hipMalloc(&c->sum, sizeof(double))) != hipSuccess)
     return bail("hipMalloc(sum)", e);
+  size_t nlam = (size_t)c->nelem * 6 * (c->p + 1) * (c->p + 1);
+  if ((e = hipMalloc(&c->lam, nlam * sizeof(double))) != hipSuccess)
+    return bail("hipMalloc(lam)", e);
   if ((e = hipMalloc(&c->status, sizeof(int))) != hipSuccess)
     return bail("hipMalloc(status)", e);
-  c->bytes = 5 * sb + nn * sizeof(double) + (size_t)c->npartial * sizeof(double) + 16;
+  c->bytes = 5 * sb + (nn + nlam + (size_t)c->npartial) * sizeof(double) + 16;
   if ((e = hipMemcpyAsync(c->nodes, mesh->nodes, nn * sizeof(double), hipMemcpyHostToDevice,
                           c->stream)) != hipSuccess)
     return bail("hipMemcpy(nodes)", e);
   if ((e = hipMemsetAsync(c->U, 0, sb, c->stream)) != hipSuccess)
     return bail("hipMemset(U)", e);
+  if ((e = lift_coef_dispatch(c->p, c->q, mesh_args(c), c->lam, c->stream)) != hipSuccess)
+    return bail("lifting coefficients", e);
   if ((e = hipMemsetAsync(c->status, 0, sizeof(int), c->stream)) != hipSuccess)
     return bail("hipMemset(status)", e);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
@@ -250,7 +256,7 @@ dgx_status dgx_destroy(dgx_ctx* c)
     return DGX_OK;
   if (c->stream)
     (void)hipStreamSynchronize(c->stream);
-  double* bufs[8] = {c->U, c->Ua, c->Ub, c->acc, c->fs, c->nodes, c->partial, c->sum};
+  double* bufs[9] = {c->U, c->Ua, c->Ub, c->acc, c->fs, c->nodes, c->partial, c->sum, c->lam};
   for (double* b : bufs)
     if (b)
       (void)hipFree(b);
@@ -353,7 +359,7 @@ static hipError_t rhs(dgx_ctx* c, const double* Vin, const double* Vbase, double
   sa.cb = cb;
   sa.cf = cf;
   sa.accmode = accmode;
-  return rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->status, c->stream);
+  return rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status, c->stream);
 }
 
 static dgx_status check_status(dgx_ctx* c)
@@ -437,6 +443,14 @@ static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm)
     return e;
   // U = 1/3 U + 2/3 U2 + 2/3 dt f
   return rhs(c, c->Ub, c->U, nullptr, c->U, nullptr, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt, 0);
+}
+
+dgx_status dgx_device_copy(dgx_ctx* c, void* dst, const void* src, uint64_t bytes)
+{
+  if (!c || !dst || !src || (bytes % 16) != 0)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, copy_dispatch(src, dst, (size_t)bytes, c->stream));
+  return DGX_OK;
 }
 
 dgx_status dgx_step_async(dgx_ctx* c, int scheme, double dt, uint64_t nsteps)

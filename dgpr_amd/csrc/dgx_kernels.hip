@@ -173,22 +173,179 @@ __device__ __forceinline__ int neighbour(const MeshArgs& m, int i, int j, int k,
   return (c[2] * m.ny + c[1]) * m.nx + c[0];
 }
 
-// One full right-hand-side evaluation f_g = B3 f of EPW elements fused with
-// the RK stage combination of StageArgs.
-template <int P, int Q>
-__global__ __launch_bounds__(64) void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph,
-                                                           StageArgs sa, int* status)
+// Faces are processed in passes of whole faces, one face point per lane:
+// FPP faces of NF points fit in one 64-lane pass.
+template <int P>
+struct FaceDims
 {
-  using Dm = Dims<P, Q>;
-  constexpr int N = Dm::N, NB = Dm::NB, NF = Dm::NF, EPW = Dm::EPW, NG = Dm::NG;
-  constexpr int NFP = Dm::NFP;
+  static constexpr int N     = P + 1;
+  static constexpr int NF    = N * N;
+  static constexpr int NB    = N * N * N;
+  static constexpr int EPW   = 64 / NB;
+  static constexpr int FPP   = 64 / NF;
+  static constexpr int NFACE = EPW * 6;
+  static constexpr int NPASS = (NFACE + FPP - 1) / FPP;
+  static constexpr int RG    = (5 * NB * EPW > 20 * 64) ? 5 * NB * EPW : 20 * 64;
+};
+
+// ---- viscous flux in velocity-gradient form ---------------------------------
+// The Newtonian flux A(s, grad s) of flux.cpp:156-202 depends on the
+// conserved-variable gradient only through the velocity gradient
+//   Lv[i][d] = d u_i / d x_d = (d m_i/dx_d - u_i d rho/dx_d) / rho
+// (there is no heat flux), and every BR2 / dual-consistency argument is a
+// jump times the normal, whose velocity gradient is rank one:
+//   (U_jump (x) n)  ->  dm (x) n,   dm_i = (jump_m_i - u_i jump_rho) / rho.
+// tau(X) = mu (X + X^T) - 2/3 mu tr(X) I;  the energy row is u . tau.
+
+// w = tau(X) n for X with X n = xn, X^T n = xtn, tr X = tr
+__device__ __forceinline__ void tau_n(double mu, const double* xn, const double* xtn, double tr,
+                                      const double* n, double* w)
+{
+  double lt = -(2.0 / 3.0) * mu * tr;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    w[i] = mu * (xn[i] + xtn[i]) + lt * n[i];
+}
+
+// velocity gradient from reference-space derivatives gr[c][0..3] of
+// (rho, m) and the inverse Jacobian Ji[c][d]
+__device__ __forceinline__ void vel_grad(const double gr[3][4], const double* u, double ir,
+                                         const double Ji[3][3], double Lv[3][3])
+{
+  double dv[3][3];  // dv[c][i] = d u_i / d xi_c
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      dv[c][i] = (gr[c][1 + i] - gr[c][0] * u[i]) * ir;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      Lv[i][d] = dv[0][i] * Ji[0][d] + dv[1][i] * Ji[1][d] + dv[2][i] * Ji[2][d];
+}
+
+// One face point seen from the own side: outputs G = F - QH.n (5) and the
+// physical dual-consistency flux dcQ as a symmetric stress S (6: xx yy zz
+// xy xz yz) whose energy row is u_L . S.
+//
+// Interior (solve.cpp:393-626):
+//   F   = roe(UL, UR, n)
+//   dcQ = A(UL, (UL - (UL+UR)/2) n)            = tau(dmL/2 (x) n)
+//   QH  = (A(UL,gL) + A(UR,gR))/2 - eta (dL + dR)/2,
+//   dL  = lamL/2 A(UL, (UL-UR) n),  dR = lamR/2 A(UR, (UL-UR) n)
+// so QH.n = tau(Lv_L/2 - cL dmL(x)n) n + tau(Lv_R/2 - cR dmR(x)n) n with
+// c = eta lam / 4, plus the energy rows u_L.(...) and u_R.(...).
+// Boundary (solve.cpp:628-794): Ub = boundary state, F = F(Ub).n,
+//   dcQ = tau(dm (x) n) with jump UL - Ub,  QH.n = A(UL, gL).n - eta lamL A(Ub, (UL-Ub) n).n
+struct FacePoint
+{
+  double G[5];
+  double S[6];
+};
+
+__device__ __forceinline__ void face_point_physics(int bt, const double* UL, const double* UR,
+                                                   const double LL[3][3], const double LR[3][3],
+                                                   const double* n, double lamL, double lamR,
+                                                   const Phys& ph, FacePoint& out, bool& ok)
+{
+  const double mu = ph.mu;
+  double irL = 1.0 / UL[0];
+  double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
+  double Ub[5];
+  const bool interior = (bt == -1);
+  if (interior)
+  {
+    roe_flux(UL, UR, n, ph.gamma, out.G);
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+      Ub[r] = UR[r];
+  }
+  else
+  {
+    ok = boundary_state(bt, UL, n, ph, Ub) && ok;
+    euler_flux_n(Ub, ph.gamma, n, out.G);
+  }
+  double irB = 1.0 / Ub[0];
+  double uB[3] = {Ub[1] * irB, Ub[2] * irB, Ub[3] * irB};
+  double jr = UL[0] - Ub[0];
+  double dmL[3], dmB[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+  {
+    double jm = UL[1 + i] - Ub[1 + i];
+    dmL[i] = (jm - jr * uL[i]) * irL;
+    dmB[i] = (jm - jr * uB[i]) * irB;
+  }
+  double dnL = dmL[0] * n[0] + dmL[1] * n[1] + dmL[2] * n[2];
+  double dnB = dmB[0] * n[0] + dmB[1] * n[1] + dmB[2] * n[2];
+
+  // own-side viscous normal flux (interior: halved, with the penalty)
+  double hL = interior ? 0.5 : 1.0;
+  double cL = interior ? 0.25 * ph.eta * lamL : 0.0;
+  double xn[3], xtn[3], w[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+  {
+    xn[i]  = hL * (LL[i][0] * n[0] + LL[i][1] * n[1] + LL[i][2] * n[2]) - cL * dmL[i];
+    xtn[i] = hL * (LL[0][i] * n[0] + LL[1][i] * n[1] + LL[2][i] * n[2]) - cL * dnL * n[i];
+  }
+  double tr = hL * (LL[0][0] + LL[1][1] + LL[2][2]) - cL * dnL;
+  tau_n(mu, xn, xtn, tr, n, w);
+  double q[4] = {w[0], w[1], w[2], uL[0] * w[0] + uL[1] * w[1] + uL[2] * w[2]};
+
+  // other side: neighbour (interior) or boundary penalty
+  double cB = interior ? 0.25 * ph.eta * lamR : ph.eta * lamL;
+  double hB = interior ? 0.5 : 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+  {
+    xn[i]  = hB * (LR[i][0] * n[0] + LR[i][1] * n[1] + LR[i][2] * n[2]) - cB * dmB[i];
+    xtn[i] = hB * (LR[0][i] * n[0] + LR[1][i] * n[1] + LR[2][i] * n[2]) - cB * dnB * n[i];
+  }
+  tr = hB * (LR[0][0] + LR[1][1] + LR[2][2]) - cB * dnB;
+  tau_n(mu, xn, xtn, tr, n, w);
+  q[0] += w[0];
+  q[1] += w[1];
+  q[2] += w[2];
+  q[3] += uB[0] * w[0] + uB[1] * w[1] + uB[2] * w[2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    out.G[1 + i] -= q[i];
+  out.G[4] -= q[3];
+
+  // dual-consistency stress tau(s dmL (x) n), s = 1/2 (interior) or 1
+  double s  = interior ? 0.5 : 1.0;
+  double a0 = s * dmL[0], a1 = s * dmL[1], a2 = s * dmL[2];
+  double lt = -(2.0 / 3.0) * mu * s * dnL;
+  out.S[0] = 2.0 * mu * a0 * n[0] + lt;
+  out.S[1] = 2.0 * mu * a1 * n[1] + lt;
+  out.S[2] = 2.0 * mu * a2 * n[2] + lt;
+  out.S[3] = mu * (a0 * n[1] + a1 * n[0]);
+  out.S[4] = mu * (a0 * n[2] + a2 * n[0]);
+  out.S[5] = mu * (a1 * n[2] + a2 * n[1]);
+}
+
+// One full right-hand-side evaluation f_g = B3 f of EPW elements fused with
+// the RK stage combination of StageArgs.  lam[e][f][pt] holds the BR2
+// lifting coefficient of element e's side of its face f.
+#ifndef DGX_STAGE_WAVES_PER_EU
+#define DGX_STAGE_WAVES_PER_EU 2
+#endif
+template <int P, int Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DGX_STAGE_WAVES_PER_EU)))
+void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
+                          int* status)
+{
+  using FD = FaceDims<P>;
+  constexpr int N = FD::N, NB = FD::NB, NF = FD::NF, EPW = FD::EPW;
+  constexpr int NG = (Q + 1) * (Q + 1) * (Q + 1);
   const PTab& pt = c_ptab[P - 1];
 
   __shared__ double sV[EPW][5][NB];
-  __shared__ double sWJ[EPW][NB];
-  __shared__ double sX[EPW][NG * 3];
-  __shared__ double sR1[Dm::RGN1];       // volume flux G, then face results
-  __shared__ double sUF[NFP][10];        // face traces: own (0-4), neighbour (5-9)
+  __shared__ double sX[EPW][NG * 3];      // own geometry nodes
+  __shared__ double sXn[EPW][6][NG * 3];  // geometry nodes of the six neighbours
+  __shared__ double sR[FD::RG];  // volume flux of one direction, then one face pass
 
   const int tid = threadIdx.x;
   const int le  = tid / NB;
@@ -206,121 +363,137 @@ __global__ __launch_bounds__(64) void dgx_rhs_stage_kernel(MeshArgs mesh, Phys p
   }
   for (int x = tid; x < EPW * NG * 3; x += 64)
   {
-    int l = x / (NG * 3);
+    int l  = x / (NG * 3);
     int ee = e0 + l;
     sX[l][x % (NG * 3)] = (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
   }
+  for (int x = tid; x < EPW * 6 * NG * 3; x += 64)
+  {
+    int k  = x % (NG * 3);
+    int fl = x / (NG * 3);
+    int ee = e0 + fl / 6;
+    double v = 0.0;
+    if (ee < mesh.nelem)
+    {
+      int bt;
+      int en = neighbour(mesh, ee % mesh.nx, (ee / mesh.nx) % mesh.ny, ee / (mesh.nx * mesh.ny),
+                         fl % 6, &bt);
+      if (en >= 0)
+        v = mesh.nodes[(size_t)en * NG * 3 + k];
+    }
+    sXn[fl / 6][fl % 6][k] = v;
+  }
   __syncthreads();
 
-  const int ni = nd0 % N, nj = (nd0 / N) % N, nk = nd0 / (N * N);
-  double Ji[3][3];
+  const int idx[3] = {nd0 % N, (nd0 / N) % N, nd0 / (N * N)};
   double wJ = 1.0;
   double acc[5] = {0, 0, 0, 0, 0};
+  double Ji[3][3];
+  double H[3][5];  // physical flux Q - F at this node
 
-  // ---- volume: gradients, fluxes, contravariant flux G ----
+  // ---- volume: velocity gradient, fluxes ----
   if (vact)
   {
     double J[3][3];
-    jac_at<P, Q>(sX[le], -1, 0, ni, nj, nk, J);
+    jac_at<P, Q>(sX[le], -1, 0, idx[0], idx[1], idx[2], J);
     double det = det3(J);
     inv3(J, det, Ji);
-    wJ = pt.w[ni] * pt.w[nj] * pt.w[nk] * det;
-    sWJ[le][nd0] = wJ;
-
-    double Vq[5], Gr[3][5];
+    wJ = pt.w[idx[0]] * pt.w[idx[1]] * pt.w[idx[2]] * det;
+    double Vq[5], gr[3][4];
 #pragma unroll
     for (int r = 0; r < 5; ++r)
-    {
       Vq[r] = sV[le][r][nd0];
-      double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+    {
+      double g0 = 0.0, g1 = 0.0, g2 = 0.0;
 #pragma unroll
       for (int m = 0; m < N; ++m)
       {
-        gx += pt.Dg[ni][m] * sV[le][r][nid<N>(m, nj, nk)];
-        gy += pt.Dg[nj][m] * sV[le][r][nid<N>(ni, m, nk)];
-        gz += pt.Dg[nk][m] * sV[le][r][nid<N>(ni, nj, m)];
+        g0 += pt.Dg[idx[0]][m] * sV[le][r][nid<N>(m, idx[1], idx[2])];
+        g1 += pt.Dg[idx[1]][m] * sV[le][r][nid<N>(idx[0], m, idx[2])];
+        g2 += pt.Dg[idx[2]][m] * sV[le][r][nid<N>(idx[0], idx[1], m)];
       }
-      // physical gradient: d/dx_d = sum_c d/dxi_c * Ji[c][d]
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        Gr[d][r] = gx * Ji[0][d] + gy * Ji[1][d] + gz * Ji[2][d];
+      gr[0][r] = g0;
+      gr[1][r] = g1;
+      gr[2][r] = g2;
     }
-    double F[3][5], Qv[3][5];
-    euler_flux(Vq, ph.gamma, F);
-    ViscState vs = visc_setup(Vq);
-    visc_apply(vs, Gr, ph.mu, Qv);
+    double ir = 1.0 / Vq[0];
+    double u[3] = {Vq[1] * ir, Vq[2] * ir, Vq[3] * ir};
+    double Lv[3][3];
+    vel_grad(gr, u, ir, Ji, Lv);
+    double p = (ph.gamma - 1.0) * (Vq[4] - 0.5 * Vq[0] * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    double Ht = (Vq[4] + p) * ir;
+    double lt = -(2.0 / 3.0) * ph.mu * (Lv[0][0] + Lv[1][1] + Lv[2][2]);
 #pragma unroll
-    for (int r = 0; r < 5; ++r)
+    for (int d = 0; d < 3; ++d)
     {
-      double h0 = Qv[0][r] - F[0][r], h1 = Qv[1][r] - F[1][r], h2 = Qv[2][r] - F[2][r];
+      double tau[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        sR1[((le * 3 + c) * 5 + r) * NB + nd0] = wJ * (Ji[c][0] * h0 + Ji[c][1] * h1 + Ji[c][2] * h2);
+      for (int i = 0; i < 3; ++i)
+        tau[i] = ph.mu * (Lv[d][i] + Lv[i][d]) + ((i == d) ? lt : 0.0);
+      double md = Vq[1 + d];
+      H[d][0] = -md;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        H[d][1 + i] = tau[i] - md * u[i] - ((i == d) ? p : 0.0);
+      H[d][4] = u[0] * tau[0] + u[1] * tau[1] + u[2] * tau[2] - md * Ht;
     }
   }
-  __syncthreads();
 
-  // ---- volume divergence: acc = sum_c Dg_c^T G_c ----
-  if (vact)
+  // ---- volume divergence, one reference direction at a time ----
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
   {
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
+    if (vact)
     {
-      double s = 0.0;
 #pragma unroll
-      for (int m = 0; m < N; ++m)
-      {
-        s += pt.Dg[m][ni] * sR1[((le * 3 + 0) * 5 + r) * NB + nid<N>(m, nj, nk)];
-        s += pt.Dg[m][nj] * sR1[((le * 3 + 1) * 5 + r) * NB + nid<N>(ni, m, nk)];
-        s += pt.Dg[m][nk] * sR1[((le * 3 + 2) * 5 + r) * NB + nid<N>(ni, nj, m)];
-      }
-      acc[r] = s;
+      for (int r = 0; r < 5; ++r)
+        sR[(le * 5 + r) * NB + nd0] = wJ * (Ji[c][0] * H[0][r] + Ji[c][1] * H[1][r] + Ji[c][2] * H[2][r]);
     }
+    __syncthreads();
+    if (vact)
+    {
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+      {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+        {
+          int id2[3] = {idx[0], idx[1], idx[2]};
+          id2[c] = m;
+          s += pt.Dg[m][idx[c]] * sR[(le * 5 + r) * NB + nid<N>(id2[0], id2[1], id2[2])];
+        }
+        acc[r] += s;
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
 
-  // ---- faces, pass 1: traces and normal derivatives of both sides ----
-  constexpr int FPT = (NFP + 63) / 64;  // face points per thread
-  double dnL[FPT][5], dnR[FPT][5];
-  int nbrE[FPT], nbrBt[FPT];
-#pragma unroll
-  for (int s = 0; s < FPT; ++s)
+  // ---- faces, in passes of FPP whole faces ----
+  // per face point in sR (stride 20): pass A: UL (0-4), UR (5-9);
+  // pass B: G (0-4), normal contravariant dcQ (5-9), tangential (10-19);
+  // pass C: T = G - Dg^T H_t1 - Dg^T H_t2 replaces G.
+  bool ok = true;
+  for (int pass = 0; pass < FD::NPASS; ++pass)
   {
-    int fp = tid + 64 * s;
-    nbrE[s] = -1;
-    nbrBt[s] = 0;
-    if (fp >= NFP) continue;
-    int l = fp / (6 * NF);
-    int f = (fp / NF) % 6;
-    int a = fp % N, b = (fp / N) % N;
-    int ee = e0 + l;
-    if (ee >= mesh.nelem) continue;
-    int nd = f >> 1, hi = f & 1;
-    int ci = ee % mesh.nx, cj = (ee / mesh.nx) % mesh.ny, ck = ee / (mesh.nx * mesh.ny);
-    int bt;
-    int en = neighbour(mesh, ci, cj, ck, f, &bt);
-    nbrE[s] = en;
-    nbrBt[s] = bt;
-    double uL[5], dL[5];
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
+    const int slot = tid / NF;
+    const int gf   = pass * FD::FPP + slot;
+    const int fpt  = tid % NF;
+    const int a = fpt % N, b = fpt / N;
+    const int l = gf / 6, f = gf % 6;
+    const int ee = e0 + l;
+    const bool fact = (slot < FD::FPP) && (gf < FD::NFACE) && (ee < mesh.nelem);
+    const int nd = f >> 1, hi = f & 1;
+    const int fbase = slot * NF;
+
+    double UL[5], UR[5], dnL[4], dnR[4];
+    int en = -1, bt = -1;
+    if (fact)
     {
-      double v = 0.0, dv = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m)
-      {
-        double x = sV[l][r][line_node<N>(nd, m, a, b)];
-        v += pt.L[hi][m] * x;
-        dv += pt.Ld[hi][m] * x;
-      }
-      uL[r] = v;
-      dL[r] = dv;
-      sUF[fp][r] = v;
-      dnL[s][r] = dv;
-    }
-    if (en >= 0)
-    {
-      const double* Vn = sa.Vin + (size_t)en * 5 * NB;
+      int ci = ee % mesh.nx, cj = (ee / mesh.nx) % mesh.ny, ck = ee / (mesh.nx * mesh.ny);
+      en = neighbour(mesh, ci, cj, ck, f, &bt);
 #pragma unroll
       for (int r = 0; r < 5; ++r)
       {
@@ -328,303 +501,228 @@ __global__ __launch_bounds__(64) void dgx_rhs_stage_kernel(MeshArgs mesh, Phys p
 #pragma unroll
         for (int m = 0; m < N; ++m)
         {
-          double x = Vn[r * NB + line_node<N>(nd, m, a, b)];
-          v += pt.L[1 - hi][m] * x;
-          dv += pt.Ld[1 - hi][m] * x;
+          double x = sV[l][r][line_node<N>(nd, m, a, b)];
+          v += pt.L[hi][m] * x;
+          if (r < 4)
+            dv += pt.Ld[hi][m] * x;
         }
-        sUF[fp][5 + r] = v;
-        dnR[s][r] = dv;
+        UL[r] = v;
+        if (r < 4)
+          dnL[r] = dv;
       }
-    }
-    else
-    {
+      if (en >= 0)
+      {
+        const double* Vn = sa.Vin + (size_t)en * 5 * NB;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+        {
+          double v = 0.0, dv = 0.0;
+#pragma unroll
+          for (int m = 0; m < N; ++m)
+          {
+            double x = Vn[r * NB + line_node<N>(nd, m, a, b)];
+            v += pt.L[1 - hi][m] * x;
+            if (r < 4)
+              dv += pt.Ld[1 - hi][m] * x;
+          }
+          UR[r] = v;
+          if (r < 4)
+            dnR[r] = dv;
+        }
+      }
+      else
+      {
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          UR[r] = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          dnR[r] = 0.0;
+      }
 #pragma unroll
       for (int r = 0; r < 5; ++r)
       {
-        sUF[fp][5 + r] = 0.0;
-        dnR[s][r] = 0.0;
+        sR[tid * 20 + r]     = UL[r];
+        sR[tid * 20 + 5 + r] = UR[r];
       }
     }
-    (void)uL;
-    (void)dL;
-  }
-  __syncthreads();
+    __syncthreads();
 
-  // ---- faces, pass 2: geometry, tangential derivatives, physics ----
-  // results per face point (scaled by fJ w_a w_b):
-  //   R1[0-4]  G  = F - QH.n          (phi-weighted term)
-  //   R1[5-9]  Hn = (Ji dcQ)_normal   (normal-derivative-weighted term)
-  //   R1[10-14], R1[15-19] tangential contravariant dcQ components
-  int bad = 0;
-  double res[FPT][20];
-#pragma unroll
-  for (int s = 0; s < FPT; ++s)
-  {
-    int fp = tid + 64 * s;
-    if (fp >= NFP) continue;
-    int l = fp / (6 * NF);
-    int f = (fp / NF) % 6;
-    int a = fp % N, b = (fp / N) % N;
-    int ee = e0 + l;
-    if (ee >= mesh.nelem)
+    FacePoint fpo;
+    double hn[4], h1[4], h2[4];
+    if (fact)
     {
-#pragma unroll
-      for (int x = 0; x < 20; ++x) res[s][x] = 0.0;
-      continue;
-    }
-    int nd = f >> 1, hi = f & 1;
-    int t1 = (nd == 0) ? 1 : 0;
-    int t2 = (nd == 2) ? 1 : 2;
-    int fbase = fp - a - N * b;  // first point of this face
+      // own geometry at the face point (end hi on direction nd)
+      const int gi0 = (nd == 0) ? 0 : a;
+      const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
+      const int gi2 = (nd == 2) ? 0 : b;
+      double J[3][3], JiL[3][3];
+      jac_at<P, Q>(sX[l], nd, hi, gi0, gi1, gi2, J);
+      inv3(J, det3(J), JiL);
+      // outward normal +/- (dx/dxi_c1 x dx/dxi_c2), (c1, c2) cyclic after nd
+      double u0 = (nd == 0) ? J[0][1] : ((nd == 1) ? J[0][2] : J[0][0]);
+      double u1 = (nd == 0) ? J[1][1] : ((nd == 1) ? J[1][2] : J[1][0]);
+      double u2 = (nd == 0) ? J[2][1] : ((nd == 1) ? J[2][2] : J[2][0]);
+      double v0 = (nd == 0) ? J[0][2] : ((nd == 1) ? J[0][0] : J[0][1]);
+      double v1 = (nd == 0) ? J[1][2] : ((nd == 1) ? J[1][0] : J[1][1]);
+      double v2 = (nd == 0) ? J[2][2] : ((nd == 1) ? J[2][0] : J[2][1]);
+      double nv[3];
+      nv[0] = u1 * v2 - u2 * v1;
+      nv[1] = u2 * v0 - u0 * v2;
+      nv[2] = u0 * v1 - u1 * v0;
+      double fJ = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+      double sg = (hi ? 1.0 : -1.0) / fJ;
+      nv[0] *= sg;
+      nv[1] *= sg;
+      nv[2] *= sg;
+      double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
 
-    // own geometry at the face point
-    int gi[3];
-    gi[nd] = 0; gi[t1] = a; gi[t2] = b;
-    double J[3][3], JiL[3][3];
-    jac_at<P, Q>(sX[l], nd, hi, gi[0], gi[1], gi[2], J);
-    double detL = det3(J);
-    inv3(J, detL, JiL);
-    // outward normal: +/- (column c1 x column c2), c1,c2 cyclic after nd
-    int c1 = (nd + 1) % 3, c2 = (nd + 2) % 3;
-    double nv[3];
-    nv[0] = J[1][c1] * J[2][c2] - J[2][c1] * J[1][c2];
-    nv[1] = J[2][c1] * J[0][c2] - J[0][c1] * J[2][c2];
-    nv[2] = J[0][c1] * J[1][c2] - J[1][c1] * J[0][c2];
-    double fJ = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-    double sg = (hi ? 1.0 : -1.0) / fJ;
-    nv[0] *= sg; nv[1] *= sg; nv[2] *= sg;
-    double fw = pt.w[a] * pt.w[b];
-
-    // lifting coefficient of the own side
-    double lamL = 0.0;
+      // reference derivatives of (rho, m) of both traces
+      double grL[3][4], grR[3][4];
 #pragma unroll
-    for (int m = 0; m < N; ++m)
-      lamL += pt.L[hi][m] * pt.L[hi][m] * fw / sWJ[l][line_node<N>(nd, m, a, b)];
-    lamL *= fJ;
-
-    double UL[5], GL[3][5];
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-    {
-      UL[r] = sUF[fp][r];
-      double ref[3];
-      ref[nd] = dnL[s][r];
-      double d1 = 0.0, d2 = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m)
+      for (int r = 0; r < 4; ++r)
       {
-        d1 += pt.Dg[a][m] * sUF[fbase + m + N * b][r];
-        d2 += pt.Dg[b][m] * sUF[fbase + a + N * m][r];
-      }
-      ref[t1] = d1;
-      ref[t2] = d2;
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        GL[d][r] = ref[0] * JiL[0][d] + ref[1] * JiL[1][d] + ref[2] * JiL[2][d];
-    }
-
-    double Gf[5], dcQ[3][5];
-    ViscState vL = visc_setup(UL);
-    if (nbrE[s] >= 0)
-    {
-      // neighbour geometry: Jacobian at its face point and along its line
-      int en = nbrE[s];
-      const double* Xr = mesh.nodes + (size_t)en * NG * 3;
-      double JR[3][3], JiR[3][3];
-      jac_at<P, Q>(Xr, nd, 1 - hi, gi[0], gi[1], gi[2], JR);
-      inv3(JR, det3(JR), JiR);
-      double lamR = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m)
-      {
-        int gm[3];
-        gm[nd] = m; gm[t1] = a; gm[t2] = b;
-        double Jm[3][3];
-        jac_at<P, Q>(Xr, -1, 0, gm[0], gm[1], gm[2], Jm);
-        lamR += pt.L[1 - hi][m] * pt.L[1 - hi][m] / (pt.w[m] * det3(Jm));
-      }
-      lamR *= fJ;
-
-      double UR[5], GR[3][5];
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-      {
-        UR[r] = sUF[fp][5 + r];
-        double ref[3];
-        ref[nd] = dnR[s][r];
-        double d1 = 0.0, d2 = 0.0;
+        double d1 = 0.0, d2 = 0.0, e1 = 0.0, e2 = 0.0;
 #pragma unroll
         for (int m = 0; m < N; ++m)
         {
-          d1 += pt.Dg[a][m] * sUF[fbase + m + N * b][5 + r];
-          d2 += pt.Dg[b][m] * sUF[fbase + a + N * m][5 + r];
+          d1 += pt.Dg[a][m] * sR[(fbase + m + N * b) * 20 + r];
+          d2 += pt.Dg[b][m] * sR[(fbase + a + N * m) * 20 + r];
+          e1 += pt.Dg[a][m] * sR[(fbase + m + N * b) * 20 + 5 + r];
+          e2 += pt.Dg[b][m] * sR[(fbase + a + N * m) * 20 + 5 + r];
         }
-        ref[t1] = d1;
-        ref[t2] = d2;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          GR[d][r] = ref[0] * JiR[0][d] + ref[1] * JiR[1][d] + ref[2] * JiR[2][d];
+        grL[0][r] = (nd == 0) ? dnL[r] : d1;
+        grL[1][r] = (nd == 1) ? dnL[r] : ((nd == 0) ? d1 : d2);
+        grL[2][r] = (nd == 2) ? dnL[r] : d2;
+        grR[0][r] = (nd == 0) ? dnR[r] : e1;
+        grR[1][r] = (nd == 1) ? dnR[r] : ((nd == 0) ? e1 : e2);
+        grR[2][r] = (nd == 2) ? dnR[r] : e2;
       }
+      double irL = 1.0 / UL[0];
+      double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
+      double LL[3][3], LR[3][3];
+      vel_grad(grL, uL, irL, JiL, LL);
+      double lamR = 0.0;
+      if (en >= 0)
+      {
+        double JR[3][3], JiR[3][3];
+        jac_at<P, Q>(sXn[l][f], nd, 1 - hi, gi0, gi1, gi2, JR);
+        inv3(JR, det3(JR), JiR);
+        lamR = lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
+        double irR = 1.0 / UR[0];
+        double uR[3] = {UR[1] * irR, UR[2] * irR, UR[3] * irR};
+        vel_grad(grR, uR, irR, JiR, LR);
+      }
+      else
+      {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          LR[i][0] = LR[i][1] = LR[i][2] = 0.0;
+      }
+      face_point_physics(en >= 0 ? -1 : bt, UL, UR, LL, LR, nv, lamL, lamR, ph, fpo, ok);
 
-      double Fr[5];
-      roe_flux(UL, UR, nv, ph.gamma, Fr);
-      ViscState vR = visc_setup(UR);
-      // D = (UL - UR) n ; dcQ_L = A(UL, D/2) ; QH.n from linearity of A:
-      // QH = A(UL, GL/2 - eta lamL D/4) + A(UR, GR/2 - eta lamR D/4)
-      double Dj[3][5], argL[3][5], argR[3][5];
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-      {
-        double jmp = UL[r] - UR[r];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-        {
-          Dj[d][r]   = 0.5 * jmp * nv[d];
-          argL[d][r] = 0.5 * GL[d][r] - 0.25 * ph.eta * lamL * jmp * nv[d];
-          argR[d][r] = 0.5 * GR[d][r] - 0.25 * ph.eta * lamR * jmp * nv[d];
-        }
-      }
-      visc_apply(vL, Dj, ph.mu, dcQ);
-      double qL[5], qR[5];
-      visc_apply_n(vL, argL, ph.mu, nv, qL);
-      visc_apply_n(vR, argR, ph.mu, nv, qR);
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-        Gf[r] = Fr[r] - (qL[r] + qR[r]);
-    }
-    else
-    {
-      double Ub[5];
-      if (!boundary_state(nbrBt[s], UL, nv, ph, Ub))
-        bad = 1;
-      double Fr[5];
-      euler_flux_n(Ub, ph.gamma, nv, Fr);
-      ViscState vb = visc_setup(Ub);
-      double Dj[3][5], argL[3][5];
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-      {
-        double jmp = UL[r] - Ub[r];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          Dj[d][r] = jmp * nv[d];
-      }
-      // lifting dL = lamL A(Ub, (UL-Ub) n)  (no 1/2, A at the boundary state)
-      double Ql[3][5];
-      visc_apply(vb, Dj, ph.mu, Ql);
-      visc_apply(vL, Dj, ph.mu, dcQ);
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          argL[d][r] = GL[d][r];
-      double qL[5];
-      visc_apply_n(vL, argL, ph.mu, nv, qL);
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-      {
-        double dln = Ql[0][r] * nv[0] + Ql[1][r] * nv[1] + Ql[2][r] * nv[2];
-        Gf[r] = Fr[r] - (qL[r] - ph.eta * lamL * dln);
-      }
-    }
-    double sc = fJ * fw;
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-    {
-      double h[3];
+      // contravariant dual-consistency flux h_c = Ji[c] . dcQ (rows 1-4)
+      double sc = fJ * pt.w[a] * pt.w[b];
+      const double* S = fpo.S;
+      double Sm[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
+      double hc[3][4];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        h[c] = JiL[c][0] * dcQ[0][r] + JiL[c][1] * dcQ[1][r] + JiL[c][2] * dcQ[2][r];
-      res[s][r]      = sc * Gf[r];
-      res[s][5 + r]  = sc * h[nd];
-      res[s][10 + r] = sc * h[t1];
-      res[s][15 + r] = sc * h[t2];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < FPT; ++s)
-  {
-    int fp = tid + 64 * s;
-    if (fp >= NFP) continue;
-#pragma unroll
-    for (int x = 0; x < 20; ++x)
-      sR1[fp * 20 + x] = res[s][x];
-  }
-  __syncthreads();
-
-  // ---- faces, pass 3: T = G - Dg^T(t1) H_t1 - Dg^T(t2) H_t2 ----
-#pragma unroll
-  for (int s = 0; s < FPT; ++s)
-  {
-    int fp = tid + 64 * s;
-    if (fp >= NFP) continue;
-    int a = fp % N, b = (fp / N) % N;
-    int fbase = fp - a - N * b;
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-    {
-      double t = res[s][r];
-#pragma unroll
-      for (int m = 0; m < N; ++m)
       {
-        t -= pt.Dg[m][a] * sR1[(fbase + m + N * b) * 20 + 10 + r];
-        t -= pt.Dg[m][b] * sR1[(fbase + a + N * m) * 20 + 15 + r];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          hc[c][i] = sc * (JiL[c][0] * Sm[0][i] + JiL[c][1] * Sm[1][i] + JiL[c][2] * Sm[2][i]);
+        hc[c][3] = uL[0] * hc[c][0] + uL[1] * hc[c][1] + uL[2] * hc[c][2];
       }
-      res[s][r] = t;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        hn[r] = (nd == 0) ? hc[0][r] : ((nd == 1) ? hc[1][r] : hc[2][r]);
+        h1[r] = (nd == 0) ? hc[1][r] : hc[0][r];
+        h2[r] = (nd == 2) ? hc[1][r] : hc[2][r];
+      }
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+        fpo.G[r] *= sc;
     }
-  }
-  __syncthreads();
+    __syncthreads();
+    if (fact)
+    {
 #pragma unroll
-  for (int s = 0; s < FPT; ++s)
-  {
-    int fp = tid + 64 * s;
-    if (fp >= NFP) continue;
+      for (int r = 0; r < 5; ++r)
+        sR[tid * 20 + r] = fpo.G[r];
 #pragma unroll
-    for (int r = 0; r < 5; ++r)
-      sR1[fp * 20 + r] = res[s][r];
+      for (int r = 0; r < 4; ++r)
+      {
+        sR[tid * 20 + 5 + r]  = hn[r];
+        sR[tid * 20 + 10 + r] = h1[r];
+        sR[tid * 20 + 15 + r] = h2[r];
+      }
+    }
+    __syncthreads();
+    // T = G - Dg^T(t1) H_t1 - Dg^T(t2) H_t2 (mass row has no viscous part);
+    // only slots 10-18 of other points are read, so T overwrites 0-4
+    if (fact)
+    {
+#pragma unroll
+      for (int r = 1; r < 5; ++r)
+      {
+        double t = fpo.G[r];
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+        {
+          t -= pt.Dg[m][a] * sR[(fbase + m + N * b) * 20 + 9 + r];
+          t -= pt.Dg[m][b] * sR[(fbase + a + N * m) * 20 + 14 + r];
+        }
+        sR[tid * 20 + r] = t;
+      }
+    }
+    __syncthreads();
+    // gather this pass's faces at the nodes
+    if (vact)
+    {
+#pragma unroll
+      for (int ff = 0; ff < 6; ++ff)
+      {
+        int g = le * 6 + ff;
+        if (g / FD::FPP != pass)
+          continue;
+        int fnd = ff >> 1, fhi = ff & 1;
+        int ft1 = (fnd == 0) ? 1 : 0;
+        int ft2 = (fnd == 2) ? 1 : 2;
+        int m  = idx[fnd];
+        int fp = (g % FD::FPP) * NF + idx[ft2] * N + idx[ft1];
+        acc[0] += pt.L[fhi][m] * sR[fp * 20];
+#pragma unroll
+        for (int r = 1; r < 5; ++r)
+          acc[r] += pt.L[fhi][m] * sR[fp * 20 + r] - pt.Ld[fhi][m] * sR[fp * 20 + 4 + r];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
 
-  if (bad)
+  if (!ok)
     atomicOr(status, 1);
 
-  // ---- gather face terms at the nodes, form f_g, RK epilogue ----
+  // ---- form f_g and the RK epilogue ----
   if (vact)
   {
-    int idx[3] = {ni, nj, nk};
-    double fg[5];
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-    {
-      double s = acc[r];
-#pragma unroll
-      for (int f = 0; f < 6; ++f)
-      {
-        int nd = f >> 1, hi = f & 1;
-        int t1 = (nd == 0) ? 1 : 0;
-        int t2 = (nd == 2) ? 1 : 2;
-        int m = idx[nd], a = idx[t1], b = idx[t2];
-        int fp = (le * 6 + f) * NF + b * N + a;
-        s += pt.L[hi][m] * sR1[fp * 20 + r] - pt.Ld[hi][m] * sR1[fp * 20 + 5 + r];
-      }
-      fg[r] = ph.source[r] - s / wJ;
-    }
+    double iwJ = 1.0 / wJ;
     size_t base = (size_t)e * 5 * NB + nd0;
 #pragma unroll
     for (int r = 0; r < 5; ++r)
     {
       size_t ix = base + (size_t)r * NB;
-      double f = fg[r];
+      double fv = ph.source[r] - acc[r] * iwJ;
       if (sa.fout)
-        sa.fout[ix] = f;
-      double ft = f;
+        sa.fout[ix] = fv;
+      double ft = fv;
       if (sa.accmode == 1)
-        sa.acc[ix] = f;
+        sa.acc[ix] = fv;
       else if (sa.accmode == 2)
-        sa.acc[ix] = sa.acc[ix] + 2.0 * f;
+        sa.acc[ix] = sa.acc[ix] + 2.0 * fv;
       else if (sa.accmode == 3)
-        ft = (sa.acc[ix] + f) * (1.0 / 6.0);
+        ft = (sa.acc[ix] + fv) * (1.0 / 6.0);
       if (sa.Vout)
       {
         double o = sa.cf * ft;
@@ -636,6 +734,51 @@ __global__ __launch_bounds__(64) void dgx_rhs_stage_kernel(MeshArgs mesh, Phys p
       }
     }
   }
+}
+
+// BR2 lifting coefficient of each element's side of each of its faces:
+//   lam[e][f][pt] = fJ(pt) * sum_m L_hi[m]^2 / (w_m detJ(line node m))
+// (the pointwise form of Phi_f M_e^-1 Phi_f^T W_f; geometry only).
+template <int P, int Q>
+__global__ __launch_bounds__(64) void dgx_lift_coef_kernel(MeshArgs mesh, double* lam)
+{
+  constexpr int N = P + 1, NF = N * N, NG = (Q + 1) * (Q + 1) * (Q + 1);
+  const PTab& pt = c_ptab[P - 1];
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t total = (size_t)mesh.nelem * 6 * NF;
+  if (t >= total)
+    return;
+  int fpt = (int)(t % NF);
+  int f   = (int)((t / NF) % 6);
+  size_t e = t / (6 * NF);
+  int a = fpt % N, b = fpt / N;
+  int nd = f >> 1, hi = f & 1;
+  int t1 = (nd == 0) ? 1 : 0;
+  int t2 = (nd == 2) ? 1 : 2;
+  const double* X = mesh.nodes + e * NG * 3;
+  int gi[3];
+  gi[nd] = 0;
+  gi[t1] = a;
+  gi[t2] = b;
+  double J[3][3];
+  jac_at<P, Q>(X, nd, hi, gi[0], gi[1], gi[2], J);
+  int c1 = (nd + 1) % 3, c2 = (nd + 2) % 3;
+  double n0 = J[1][c1] * J[2][c2] - J[2][c1] * J[1][c2];
+  double n1 = J[2][c1] * J[0][c2] - J[0][c1] * J[2][c2];
+  double n2 = J[0][c1] * J[1][c2] - J[1][c1] * J[0][c2];
+  double fJ = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  double s = 0.0;
+  for (int m = 0; m < N; ++m)
+  {
+    int gm[3];
+    gm[nd] = m;
+    gm[t1] = a;
+    gm[t2] = b;
+    double Jm[3][3];
+    jac_at<P, Q>(X, -1, 0, gm[0], gm[1], gm[2], Jm);
+    s += pt.L[hi][m] * pt.L[hi][m] / (pt.w[m] * det3(Jm));
+  }
+  lam[t] = fJ * s;
 }
 
 // ---- basis conversions and residual output ------------------------------
@@ -876,16 +1019,47 @@ __global__ __launch_bounds__(256) void dgx_sum_kernel(const double* partial, int
     out[0] = red[0];
 }
 
+// Plain streaming copy (16 B per lane per iteration).
+__global__ __launch_bounds__(256) void dgx_copy_kernel(const double2* __restrict__ src,
+                                                      double2* __restrict__ dst, size_t n2)
+{
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride)
+    dst[i] = src[i];
+}
+
+hipError_t copy_dispatch(const void* src, void* dst, size_t bytes, hipStream_t st)
+{
+  size_t n2 = bytes / 16;
+  int nblk = (int)((n2 + 255) / 256);
+  if (nblk > 8192)
+    nblk = 8192;
+  if (nblk < 1)
+    nblk = 1;
+  hipLaunchKernelGGL(dgx_copy_kernel, dim3(nblk), dim3(256), 0, st, (const double2*)src,
+                     (double2*)dst, n2);
+  return hipGetLastError();
+}
+
 // ---- launch helpers (host) -----------------------------------------------
 
 template <int P, int Q>
-hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, int* status,
-                      hipStream_t st)
+hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
+                      int* status, hipStream_t st)
 {
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
-  hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, sa,
+  hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, sa, lam,
                      status);
+  return hipGetLastError();
+}
+
+template <int P, int Q>
+hipError_t launch_lift_coef(const MeshArgs& m, double* lam, hipStream_t st)
+{
+  size_t total = (size_t)m.nelem * 6 * (P + 1) * (P + 1);
+  int nblk = (int)((total + 63) / 64);
+  hipLaunchKernelGGL((dgx_lift_coef_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, lam);
   return hipGetLastError();
 }
 
@@ -932,11 +1106,18 @@ int resid_out_blocks(int p, int nelem)
   return hipErrorInvalidValue;
 
 hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
-                        int* status, hipStream_t st)
+                        const double* lam, int* status, hipStream_t st)
 {
-#define CALL_RHS(P, Q) launch_rhs<P, Q>(m, ph, sa, status, st)
+#define CALL_RHS(P, Q) launch_rhs<P, Q>(m, ph, sa, lam, status, st)
   DGX_DISPATCH_PQ(p, q, CALL_RHS)
 #undef CALL_RHS
+}
+
+hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipStream_t st)
+{
+#define CALL_LC(P, Q) launch_lift_coef<P, Q>(m, lam, st)
+  DGX_DISPATCH_PQ(p, q, CALL_LC)
+#undef CALL_LC
 }
 
 hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg, double* f,

@@ -69,13 +69,16 @@ struct StageArgs
 };
 
 hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
-                        int* status, hipStream_t st);
+                        const double* lam, int* status, hipStream_t st);
+// lam[e][f][(p+1)^2]: BR2 lifting coefficient of each element side of each face
+hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipStream_t st);
 hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg, double* f,
                               double* R, double* partial, hipStream_t st);
 hipError_t basis_dispatch(int p, const double* in, double* out, int nelem, int mode,
                           hipStream_t st);
 hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st);
 hipError_t upload_tables(const PTab* pt, const QTab* qt);
+hipError_t copy_dispatch(const void* src, void* dst, size_t bytes, hipStream_t st);
 int resid_out_blocks(int p, int nelem);
 
 namespace host

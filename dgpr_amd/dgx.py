@@ -91,6 +91,7 @@ def lib():
         L.dgx_residual_device.argtypes = [P_ctx, C.c_void_p, C.c_void_p, C.c_void_p]
         L.dgx_step.argtypes = [P_ctx, C.c_int, D, U64, PD]
         L.dgx_step_async.argtypes = [P_ctx, C.c_int, D, U64]
+        L.dgx_device_copy.argtypes = [P_ctx, C.c_void_p, C.c_void_p, U64]
         L.dgx_sync.argtypes = [P_ctx]
         L.dgx_get_stream.argtypes = [P_ctx, C.POINTER(C.c_void_p)]
         L.dgx_get_sizes.argtypes = [P_ctx, C.POINTER(U64)]
@@ -106,7 +107,7 @@ def lib():
         L.dgx_read_dg.argtypes = [C.c_char_p, PD, PD]
         for nm in ("dgx_create", "dgx_destroy", "dgx_set_params", "dgx_set_state",
                    "dgx_get_state", "dgx_set_state_device", "dgx_get_state_device",
-                   "dgx_residual", "dgx_residual_device", "dgx_step", "dgx_step_async",
+                   "dgx_residual", "dgx_residual_device", "dgx_step", "dgx_step_async", "dgx_device_copy",
                    "dgx_sync", "dgx_get_stream", "dgx_get_sizes", "dgx_make_reference_state",
                    "dgx_bump_nodes", "dgx_box_nodes", "dgx_tgv_state", "dgx_channel_state",
                    "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg"):

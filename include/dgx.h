@@ -148,6 +148,14 @@ dgx_status dgx_step(dgx_ctx* ctx, int scheme, double dt, uint64_t nsteps,
 dgx_status dgx_step_async(dgx_ctx* ctx, int scheme, double dt,
                           uint64_t nsteps);
 
+/* Enqueues a device-to-device copy of bytes (a multiple of 16) on the
+ * context's stream with a plain streaming kernel.  Replaces the reference's
+ * state copies between device buffers (cudaMemcpy device-to-device is not
+ * used by the reference, which copies through the host, cudg.cu:115-136);
+ * it also serves as the known-size kernel for memory-traffic calibration. */
+dgx_status dgx_device_copy(dgx_ctx* ctx, void* dst, const void* src,
+                           uint64_t bytes);
+
 /* Waits for all work on the context's stream. */
 dgx_status dgx_sync(dgx_ctx* ctx);
 

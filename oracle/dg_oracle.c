@@ -53,6 +53,25 @@ static void* zalloc(size_t n)
   return calloc(1, n ? n : 1);
 }
 
+/* Planted mistakes, for tests that show an invariant catches a plausible
+ * error (never set outside those tests):
+ *   DC_SIGN      R-side dual-consistency term with the opposite sign
+ *   DROP_RFLUX   R side misses the Roe flux term
+ *   RK4_HALF     classic RK4 stage 3 with dt/2 instead of dt */
+enum
+{
+  ORC_PLANT_NONE       = 0,
+  ORC_PLANT_DC_SIGN    = 1,
+  ORC_PLANT_DROP_RFLUX = 2,
+  ORC_PLANT_RK4_HALF   = 3
+};
+static int g_plant = ORC_PLANT_NONE;
+
+void orc_set_plant(int k)
+{
+  g_plant = k;
+}
+
 /* btype values, boundaries.cpp:26-35 */
 enum
 {
@@ -1323,10 +1342,11 @@ int orc_residual(const orc_geom* g, const orc_params* prm, const double* U,
         {
           double tR = fevR[nbf * qi + t];
           const double* gR = g->fgrad + ((u64)nbf * (nfq * (6 * eR + lfR) + qi) + t) * 3;
-          R[SIDX(eR, r, t)] += (-(tR * F[r]) -
-                                (gR[0] * dcQRx[r] + gR[1] * dcQRy[r] + gR[2] * dcQRz[r]) +
-                                (tR * QH[r])) *
-                               se * qw;
+          double dcR = gR[0] * dcQRx[r] + gR[1] * dcQRy[r] + gR[2] * dcQRz[r];
+          if (g_plant == ORC_PLANT_DC_SIGN)
+            dcR = -dcR;
+          double FR = (g_plant == ORC_PLANT_DROP_RFLUX) ? 0.0 : tR * F[r];
+          R[SIDX(eR, r, t)] += (-FR - dcR + (tR * QH[r])) * se * qw;
         }
     }
   }
@@ -1475,7 +1495,10 @@ int orc_step(const orc_geom* g, const orc_params* prm, int scheme, double dt,
     st |= orc_residual(g, prm, U1, R, f2);
     for (u64 i = 0; i < N; ++i) U2[i] = U[i] + dt * f2[i] * 0.5;
     st |= orc_residual(g, prm, U2, R, f3);
-    for (u64 i = 0; i < N; ++i) U3[i] = U[i] + dt * f3[i];
+    {
+      double c3 = (g_plant == ORC_PLANT_RK4_HALF) ? 0.5 : 1.0;
+      for (u64 i = 0; i < N; ++i) U3[i] = U[i] + dt * f3[i] * c3;
+    }
     st |= orc_residual(g, prm, U3, R, f4);
     for (u64 i = 0; i < N; ++i)
       U[i] = U[i] + (1. / 6.) * (f1[i] + 2. * f2[i] + 2. * f3[i] + f4[i]) * dt;
@@ -1518,7 +1541,10 @@ void orc_step_generic(int scheme, double dt, u64 N, double* U, orc_rhs_fn rhs,
     rhs(U1, f2, ctx);
     for (u64 i = 0; i < N; ++i) U2[i] = U[i] + dt * f2[i] * 0.5;
     rhs(U2, f3, ctx);
-    for (u64 i = 0; i < N; ++i) U3[i] = U[i] + dt * f3[i];
+    {
+      double c3 = (g_plant == ORC_PLANT_RK4_HALF) ? 0.5 : 1.0;
+      for (u64 i = 0; i < N; ++i) U3[i] = U[i] + dt * f3[i] * c3;
+    }
     rhs(U3, f4, ctx);
     for (u64 i = 0; i < N; ++i)
       U[i] = U[i] + (1. / 6.) * (f1[i] + 2. * f2[i] + 2. * f3[i] + f4[i]) * dt;

@@ -74,6 +74,7 @@ def lib():
         L.orc_step.restype = C.c_int
         L.orc_residual_norm.argtypes = [PD, U64]
         L.orc_residual_norm.restype = D
+        L.orc_set_plant.argtypes = [C.c_int]
         L.RHS = C.CFUNCTYPE(None, PD, PD, C.c_void_p)
         L.orc_step_generic.argtypes = [C.c_int, D, U64, PD, L.RHS, C.c_void_p]
     return _lib
@@ -325,3 +326,30 @@ def step_generic(scheme, dt, U, rhs):
     cb = L.RHS(_cb)
     L.orc_step_generic(0 if scheme == "RK4" else 1, dt, n, _p(U), cb, None)
     return U
+
+
+class planted:
+    """Context manager planting a deliberate mistake in the oracle (tests of
+    the tests only): 1 = R-side dual-consistency sign, 2 = R side drops the
+    Roe flux, 3 = RK4 stage 3 with dt/2."""
+
+    def __init__(self, kind):
+        self.kind = kind
+
+    def __enter__(self):
+        lib().orc_set_plant(self.kind)
+        return self
+
+    def __exit__(self, *a):
+        lib().orc_set_plant(0)
+
+
+def params_from(prm):
+    """oracle Params from any object with the reference ``parameters`` fields."""
+    o = Params()
+    for f in ("gamma", "R", "mu", "c", "T", "P", "eta"):
+        setattr(o, f, getattr(prm, f))
+    for i in range(5):
+        o.Ufr[i] = prm.Ufr[i]
+        o.source[i] = prm.source[i]
+    return o

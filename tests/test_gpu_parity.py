@@ -20,13 +20,7 @@ SOL_TOL = 1e-10
 
 
 def orc_params(orc, prm):
-    o = orc.Params()
-    for f in ("gamma", "R", "mu", "c", "T", "P", "eta"):
-        setattr(o, f, getattr(prm, f))
-    for i in range(5):
-        o.Ufr[i] = prm.Ufr[i]
-        o.source[i] = prm.source[i]
-    return o
+    return orc.params_from(prm)
 
 
 def rel(a, b):
