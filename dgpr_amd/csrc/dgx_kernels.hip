@@ -150,6 +150,28 @@ __device__ __forceinline__ void jac_at(const double* X, int nd, int e, int i, in
   jacobian<Q>(X, v[0], d[0], v[1], d[1], v[2], d[2], J);
 }
 
+// Outward unit normal of local face (nd, hi) from the Jacobian at a face
+// point: +/- (dx/dxi_c1 x dx/dxi_c2) with (c1, c2) cyclic after nd, negated
+// on low faces (geometry_common.cpp:416-454).  Returns the surface element.
+__device__ __forceinline__ double face_normal(const double J[3][3], int nd, int hi, double* nv)
+{
+  double u0 = (nd == 0) ? J[0][1] : ((nd == 1) ? J[0][2] : J[0][0]);
+  double u1 = (nd == 0) ? J[1][1] : ((nd == 1) ? J[1][2] : J[1][0]);
+  double u2 = (nd == 0) ? J[2][1] : ((nd == 1) ? J[2][2] : J[2][0]);
+  double v0 = (nd == 0) ? J[0][2] : ((nd == 1) ? J[0][0] : J[0][1]);
+  double v1 = (nd == 0) ? J[1][2] : ((nd == 1) ? J[1][0] : J[1][1]);
+  double v2 = (nd == 0) ? J[2][2] : ((nd == 1) ? J[2][0] : J[2][1]);
+  nv[0] = u1 * v2 - u2 * v1;
+  nv[1] = u2 * v0 - u0 * v2;
+  nv[2] = u0 * v1 - u1 * v0;
+  double fJ = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+  double sg = (hi ? 1.0 : -1.0) / fJ;
+  nv[0] *= sg;
+  nv[1] *= sg;
+  nv[2] *= sg;
+  return fJ;
+}
+
 // Neighbour of element (i,j,k) across local face f (0:-x 1:+x 2:-y 3:+y
 // 4:-z 5:+z) on the structured mesh; returns -1 with *bt set for a
 // non-periodic domain boundary (geometry_shared.cpp:199-229).
@@ -244,28 +266,52 @@ struct FacePoint
   double S[6];
 };
 
-__device__ __forceinline__ void face_point_physics(int bt, const double* UL, const double* UR,
-                                                   const double LL[3][3], const double LR[3][3],
-                                                   const double* n, double lamL, double lamR,
-                                                   const Phys& ph, FacePoint& out, bool& ok)
+// Gradient traction tau(grad u).n of one side, from the reference-space
+// derivatives gr[c][0..3] of (rho, m), the inverse Jacobian Ji and the normal:
+// with dv[c][i] = d u_i/d xi_c,  (L n)_i = sum_c dv[c][i] (Ji n)_c,
+// (L^T n)_d = sum_c Ji[c][d] (dv[c] . n),  tr L = sum_c dv[c] . Ji[c].
+__device__ __forceinline__ void grad_traction(const double gr[3][4], const double* U,
+                                              const double Ji[3][3], const double* n, double mu,
+                                              double* vg)
+{
+  double ir = 1.0 / U[0];
+  double u[3] = {U[1] * ir, U[2] * ir, U[3] * ir};
+  double Ln[3] = {0.0, 0.0, 0.0}, LTn[3] = {0.0, 0.0, 0.0}, tr = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+  {
+    double dv0 = (gr[c][1] - gr[c][0] * u[0]) * ir;
+    double dv1 = (gr[c][2] - gr[c][0] * u[1]) * ir;
+    double dv2 = (gr[c][3] - gr[c][0] * u[2]) * ir;
+    double mc = Ji[c][0] * n[0] + Ji[c][1] * n[1] + Ji[c][2] * n[2];
+    double kc = dv0 * n[0] + dv1 * n[1] + dv2 * n[2];
+    Ln[0] += dv0 * mc;
+    Ln[1] += dv1 * mc;
+    Ln[2] += dv2 * mc;
+    LTn[0] += Ji[c][0] * kc;
+    LTn[1] += Ji[c][1] * kc;
+    LTn[2] += Ji[c][2] * kc;
+    tr += dv0 * Ji[c][0] + dv1 * Ji[c][1] + dv2 * Ji[c][2];
+  }
+  tau_n(mu, Ln, LTn, tr, n, vg);
+}
+
+// vgL, vgR: gradient tractions tau(grad u).n of the own and neighbour traces
+// (vgR unused on boundary faces).  tau(a (x) n).n = mu a + mu/3 (a.n) n.
+// Ub is the neighbour trace (interior) or the boundary state (boundary).
+__device__ __forceinline__ void face_point_physics(bool interior, const double* UL,
+                                                   const double* Ub, const double* vgL,
+                                                   const double* vgR, const double* n,
+                                                   double lamL, double lamR, const Phys& ph,
+                                                   FacePoint& out)
 {
   const double mu = ph.mu;
   double irL = 1.0 / UL[0];
   double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
-  double Ub[5];
-  const bool interior = (bt == -1);
   if (interior)
-  {
-    roe_flux(UL, UR, n, ph.gamma, out.G);
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-      Ub[r] = UR[r];
-  }
+    roe_flux(UL, Ub, n, ph.gamma, out.G);
   else
-  {
-    ok = boundary_state(bt, UL, n, ph, Ub) && ok;
     euler_flux_n(Ub, ph.gamma, n, out.G);
-  }
   double irB = 1.0 / Ub[0];
   double uB[3] = {Ub[1] * irB, Ub[2] * irB, Ub[3] * irB};
   double jr = UL[0] - Ub[0];
@@ -280,31 +326,22 @@ __device__ __forceinline__ void face_point_physics(int bt, const double* UL, con
   double dnL = dmL[0] * n[0] + dmL[1] * n[1] + dmL[2] * n[2];
   double dnB = dmB[0] * n[0] + dmB[1] * n[1] + dmB[2] * n[2];
 
-  // own-side viscous normal flux (interior: halved, with the penalty)
+  // own side: hL tau(grad u_L).n - cL tau(dmL (x) n).n (interior: halved,
+  // with the penalty; boundary: the full gradient traction)
   double hL = interior ? 0.5 : 1.0;
   double cL = interior ? 0.25 * ph.eta * lamL : 0.0;
-  double xn[3], xtn[3], w[3];
+  double w[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
-  {
-    xn[i]  = hL * (LL[i][0] * n[0] + LL[i][1] * n[1] + LL[i][2] * n[2]) - cL * dmL[i];
-    xtn[i] = hL * (LL[0][i] * n[0] + LL[1][i] * n[1] + LL[2][i] * n[2]) - cL * dnL * n[i];
-  }
-  double tr = hL * (LL[0][0] + LL[1][1] + LL[2][2]) - cL * dnL;
-  tau_n(mu, xn, xtn, tr, n, w);
+    w[i] = hL * vgL[i] - cL * mu * (dmL[i] + (1.0 / 3.0) * dnL * n[i]);
   double q[4] = {w[0], w[1], w[2], uL[0] * w[0] + uL[1] * w[1] + uL[2] * w[2]};
 
-  // other side: neighbour (interior) or boundary penalty
+  // other side: neighbour (interior) or boundary penalty at the boundary state
   double cB = interior ? 0.25 * ph.eta * lamR : ph.eta * lamL;
   double hB = interior ? 0.5 : 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
-  {
-    xn[i]  = hB * (LR[i][0] * n[0] + LR[i][1] * n[1] + LR[i][2] * n[2]) - cB * dmB[i];
-    xtn[i] = hB * (LR[0][i] * n[0] + LR[1][i] * n[1] + LR[2][i] * n[2]) - cB * dnB * n[i];
-  }
-  tr = hB * (LR[0][0] + LR[1][1] + LR[2][2]) - cB * dnB;
-  tau_n(mu, xn, xtn, tr, n, w);
+    w[i] = hB * vgR[i] - cB * mu * (dmB[i] + (1.0 / 3.0) * dnB * n[i]);
   q[0] += w[0];
   q[1] += w[1];
   q[2] += w[2];
@@ -559,67 +596,68 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
       double J[3][3], JiL[3][3];
       jac_at<P, Q>(sX[l], nd, hi, gi0, gi1, gi2, J);
       inv3(J, det3(J), JiL);
-      // outward normal +/- (dx/dxi_c1 x dx/dxi_c2), (c1, c2) cyclic after nd
-      double u0 = (nd == 0) ? J[0][1] : ((nd == 1) ? J[0][2] : J[0][0]);
-      double u1 = (nd == 0) ? J[1][1] : ((nd == 1) ? J[1][2] : J[1][0]);
-      double u2 = (nd == 0) ? J[2][1] : ((nd == 1) ? J[2][2] : J[2][0]);
-      double v0 = (nd == 0) ? J[0][2] : ((nd == 1) ? J[0][0] : J[0][1]);
-      double v1 = (nd == 0) ? J[1][2] : ((nd == 1) ? J[1][0] : J[1][1]);
-      double v2 = (nd == 0) ? J[2][2] : ((nd == 1) ? J[2][0] : J[2][1]);
       double nv[3];
-      nv[0] = u1 * v2 - u2 * v1;
-      nv[1] = u2 * v0 - u0 * v2;
-      nv[2] = u0 * v1 - u1 * v0;
-      double fJ = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-      double sg = (hi ? 1.0 : -1.0) / fJ;
-      nv[0] *= sg;
-      nv[1] *= sg;
-      nv[2] *= sg;
+      double fJ = face_normal(J, nd, hi, nv);
       double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
 
-      // reference derivatives of (rho, m) of both traces
-      double grL[3][4], grR[3][4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-      {
-        double d1 = 0.0, d2 = 0.0, e1 = 0.0, e2 = 0.0;
-#pragma unroll
-        for (int m = 0; m < N; ++m)
-        {
-          d1 += pt.Dg[a][m] * sR[(fbase + m + N * b) * 20 + r];
-          d2 += pt.Dg[b][m] * sR[(fbase + a + N * m) * 20 + r];
-          e1 += pt.Dg[a][m] * sR[(fbase + m + N * b) * 20 + 5 + r];
-          e2 += pt.Dg[b][m] * sR[(fbase + a + N * m) * 20 + 5 + r];
-        }
-        grL[0][r] = (nd == 0) ? dnL[r] : d1;
-        grL[1][r] = (nd == 1) ? dnL[r] : ((nd == 0) ? d1 : d2);
-        grL[2][r] = (nd == 2) ? dnL[r] : d2;
-        grR[0][r] = (nd == 0) ? dnR[r] : e1;
-        grR[1][r] = (nd == 1) ? dnR[r] : ((nd == 0) ? e1 : e2);
-        grR[2][r] = (nd == 2) ? dnR[r] : e2;
-      }
-      double irL = 1.0 / UL[0];
-      double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
-      double LL[3][3], LR[3][3];
-      vel_grad(grL, uL, irL, JiL, LL);
+      // neighbour side first, reduced to its gradient traction
       double lamR = 0.0;
+      double vgR[3] = {0.0, 0.0, 0.0};
       if (en >= 0)
       {
         double JR[3][3], JiR[3][3];
         jac_at<P, Q>(sXn[l][f], nd, 1 - hi, gi0, gi1, gi2, JR);
         inv3(JR, det3(JR), JiR);
         lamR = lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
-        double irR = 1.0 / UR[0];
-        double uR[3] = {UR[1] * irR, UR[2] * irR, UR[3] * irR};
-        vel_grad(grR, uR, irR, JiR, LR);
+        double grR[3][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          double e1 = 0.0, e2 = 0.0;
+#pragma unroll
+          for (int m = 0; m < N; ++m)
+          {
+            e1 += pt.Dg[a][m] * sR[(fbase + m + N * b) * 20 + 5 + r];
+            e2 += pt.Dg[b][m] * sR[(fbase + a + N * m) * 20 + 5 + r];
+          }
+          grR[0][r] = (nd == 0) ? dnR[r] : e1;
+          grR[1][r] = (nd == 1) ? dnR[r] : ((nd == 0) ? e1 : e2);
+          grR[2][r] = (nd == 2) ? dnR[r] : e2;
+        }
+        grad_traction(grR, UR, JiR, nv, ph.mu, vgR);
       }
-      else
+      // own side
+      double vgL[3];
+      {
+        double grL[3][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+          for (int m = 0; m < N; ++m)
+          {
+            d1 += pt.Dg[a][m] * sR[(fbase + m + N * b) * 20 + r];
+            d2 += pt.Dg[b][m] * sR[(fbase + a + N * m) * 20 + r];
+          }
+          grL[0][r] = (nd == 0) ? dnL[r] : d1;
+          grL[1][r] = (nd == 1) ? dnL[r] : ((nd == 0) ? d1 : d2);
+          grL[2][r] = (nd == 2) ? dnL[r] : d2;
+        }
+        grad_traction(grL, UL, JiL, nv, ph.mu, vgL);
+      }
+      double irL = 1.0 / UL[0];
+      double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
+      double Ub[5];
+      if (en >= 0)
       {
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-          LR[i][0] = LR[i][1] = LR[i][2] = 0.0;
+        for (int r = 0; r < 5; ++r)
+          Ub[r] = UR[r];
       }
-      face_point_physics(en >= 0 ? -1 : bt, UL, UR, LL, LR, nv, lamL, lamR, ph, fpo, ok);
+      else
+        ok = boundary_state(bt, UL, nv, ph, Ub) && ok;
+      face_point_physics(en >= 0, UL, Ub, vgL, vgR, nv, lamL, lamR, ph, fpo);
 
       // contravariant dual-consistency flux h_c = Ji[c] . dcQ (rows 1-4)
       double sc = fJ * pt.w[a] * pt.w[b];
