@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--workload", default="config1", choices=["config0", "config1", "config2"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default occupancy target instead of timing all of them")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one copy kernel of this size (traffic calibration)")
     a = ap.parse_args()
@@ -106,6 +108,13 @@ def main():
     s.set_state(case.U0)
     stream = C.c_void_p(s.stream())
     s.step_async(case.dt, case.scheme, a.warmup)
+    s.sync()
+    # launch tuning (untimed): each compiled occupancy target on this mesh
+    if a.autotune:
+        waves, tune_us = s.autotune(reps=3)
+    else:
+        waves, tune_us = s.stage_waves, None
+    s.step_async(case.dt, case.scheme, 1)
     s.sync()
     h.hipDeviceSynchronize()
 
@@ -176,10 +185,12 @@ def main():
             "nelem": case.mesh.nelem, "p": case.mesh.p, "q": case.mesh.q,
             "dof": case.dof, "scheme": case.scheme, "dt": case.dt,
             "parallelism": "single GPU",
+            "stage_waves_per_simd": waves,
+            "autotune_us": None if tune_us is None else [round(float(x), 2) for x in tune_us],
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "dgx_rhs_stage_kernel<%d,%d>" % (case.mesh.p, case.mesh.q),
+            "kernel": "dgx_rhs_stage_kernel<%d,%d,%d>" % (case.mesh.p, case.mesh.q, waves),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

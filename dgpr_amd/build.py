@@ -32,7 +32,7 @@ def _run(cmd, verbose):
 
 def build(verbose: bool = False, jobs: int = 4, out: str = OUT, defines=()) -> str:
     """Compile every source and link ``out``; ``defines`` are extra -D flags
-    (kernel tuning parameters such as DGX_STAGE_WAVES_PER_EU=2)."""
+    (for experiments; the shipped library needs none)."""
     build_dir = BUILD if out == OUT else out + ".objs"
     os.makedirs(build_dir, exist_ok=True)
     objs = []

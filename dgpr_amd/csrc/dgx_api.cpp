@@ -39,6 +39,11 @@ struct dgx_ctx
   double* partial;  // per-block sums of R^2
   double* sum;      // reduced sum
   double* lam;      // BR2 lifting coefficients [e][6][(p+1)^2]
+  double* TU;       // face trace records of U, Ua, Ub [e][6][8][(p+1)^2]
+  double* TUa;
+  double* TUb;
+  bool tu_valid;    // TU holds the traces of the current U
+  int stage_waves;  // occupancy target of the stage kernel (1..3)
   int* status;      // kernel-side error flags
   int npartial;
   size_t bytes;
@@ -66,6 +71,13 @@ static dgx_status fail(dgx_ctx* c, dgx_status st, const char* fmt, ...)
     if (e_ != hipSuccess)                                                                  \
       return fail((ctx), DGX_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
   } while (0)
+
+// Occupancy target per (p, q): 3 waves/SIMD where the stage kernel fits the
+// 168-register budget without spilling (p=3, q=1), 2 elsewhere.
+static int default_stage_waves(int p, int q)
+{
+  return (p == 3 && q == 1) ? 3 : 2;
+}
 
 static void derive_phys(const dgx_params& p, Phys& ph)
 {
@@ -234,7 +246,14 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
     return bail("hipMalloc(lam)", e);
   if ((e = hipMalloc(&c->status, sizeof(int))) != hipSuccess)
     return bail("hipMalloc(status)", e);
-  c->bytes = 5 * sb + (nn + nlam + (size_t)c->npartial) * sizeof(double) + 16;
+  size_t tb = nlam * 8 * sizeof(double);
+  double** tbufs[3] = {&c->TU, &c->TUa, &c->TUb};
+  for (int i = 0; i < 3; ++i)
+    if ((e = hipMalloc(tbufs[i], tb)) != hipSuccess)
+      return bail("hipMalloc(traces)", e);
+  c->tu_valid = false;
+  c->stage_waves = default_stage_waves(c->p, c->q);
+  c->bytes = 5 * sb + 3 * tb + (nn + nlam + (size_t)c->npartial) * sizeof(double) + 16;
   if ((e = hipMemcpyAsync(c->nodes, mesh->nodes, nn * sizeof(double), hipMemcpyHostToDevice,
                           c->stream)) != hipSuccess)
     return bail("hipMemcpy(nodes)", e);
@@ -256,7 +275,8 @@ dgx_status dgx_destroy(dgx_ctx* c)
     return DGX_OK;
   if (c->stream)
     (void)hipStreamSynchronize(c->stream);
-  double* bufs[9] = {c->U, c->Ua, c->Ub, c->acc, c->fs, c->nodes, c->partial, c->sum, c->lam};
+  double* bufs[12] = {c->U,       c->Ua,  c->Ub,  c->acc, c->fs,  c->nodes,
+                      c->partial, c->sum, c->lam, c->TU,  c->TUa, c->TUb};
   for (double* b : bufs)
     if (b)
       (void)hipFree(b);
@@ -276,6 +296,7 @@ dgx_status dgx_set_params(dgx_ctx* c, const dgx_params* params)
     return fail(c, DGX_ERR_INVALID, "invalid reference state");
   c->params = *params;
   derive_phys(*params, c->phys);
+  c->tu_valid = false;  // the gradient tractions scale with mu
   return DGX_OK;
 }
 
@@ -311,6 +332,7 @@ dgx_status dgx_set_state_device(dgx_ctx* c, const double* U_dev)
   if (!c || !U_dev)
     return DGX_ERR_INVALID;
   HIPCHK(c, basis_dispatch(c->p, U_dev, c->U, c->nelem, 0, c->stream));
+  c->tu_valid = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DGX_OK;
 }
@@ -331,6 +353,7 @@ dgx_status dgx_set_state(dgx_ctx* c, const double* U_host)
   HIPCHK(c, hipMemcpyAsync(c->Ua, U_host, c->nstate * sizeof(double), hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, basis_dispatch(c->p, c->Ua, c->U, c->nelem, 0, c->stream));
+  c->tu_valid = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DGX_OK;
 }
@@ -346,8 +369,11 @@ dgx_status dgx_get_state(dgx_ctx* c, double* U_host)
   return DGX_OK;
 }
 
-static hipError_t rhs(dgx_ctx* c, const double* Vin, const double* Vbase, double* acc,
-                      double* Vout, double* fout, double ca, double cb, double cf, int accmode)
+// One stage: Vin with its trace records Tin; Tout (with Vout) receives the
+// trace records of Vout.
+static hipError_t rhs(dgx_ctx* c, const double* Vin, const double* Tin, const double* Vbase,
+                      double* acc, double* Vout, double* Tout, double* fout, double ca, double cb,
+                      double cf, int accmode)
 {
   StageArgs sa;
   sa.Vin = Vin;
@@ -355,11 +381,25 @@ static hipError_t rhs(dgx_ctx* c, const double* Vin, const double* Vbase, double
   sa.acc = acc;
   sa.Vout = Vout;
   sa.fout = fout;
+  sa.Tin = Tin;
+  sa.Tout = Tout;
   sa.ca = ca;
   sa.cb = cb;
   sa.cf = cf;
   sa.accmode = accmode;
-  return rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status, c->stream);
+  return rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status, c->stage_waves,
+                      c->stream);
+}
+
+// Trace records of U, recomputed when U was set from outside.
+static hipError_t ensure_traces(dgx_ctx* c)
+{
+  if (c->tu_valid)
+    return hipSuccess;
+  hipError_t e = trace_dispatch(c->p, c->q, mesh_args(c), c->phys.mu, c->U, c->TU, c->stream);
+  if (e == hipSuccess)
+    c->tu_valid = true;
+  return e;
 }
 
 static dgx_status check_status(dgx_ctx* c)
@@ -380,7 +420,8 @@ dgx_status dgx_residual_device(dgx_ctx* c, const double* U_dev, double* R_dev, d
   if (!c || !U_dev || !f_dev)
     return DGX_ERR_INVALID;
   HIPCHK(c, basis_dispatch(c->p, U_dev, c->Ua, c->nelem, 0, c->stream));
-  HIPCHK(c, rhs(c, c->Ua, nullptr, nullptr, nullptr, c->fs, 0, 0, 0, 0));
+  HIPCHK(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys.mu, c->Ua, c->TUa, c->stream));
+  HIPCHK(c, rhs(c, c->Ua, c->TUa, nullptr, nullptr, nullptr, nullptr, c->fs, 0, 0, 0, 0));
   HIPCHK(c, resid_out_dispatch(c->p, c->q, mesh_args(c), c->fs, f_dev, R_dev, nullptr,
                                c->stream));
   return check_status(c);
@@ -390,7 +431,8 @@ dgx_status dgx_residual(dgx_ctx* c, double* R_host, double* f_host)
 {
   if (!c || !f_host)
     return DGX_ERR_INVALID;
-  HIPCHK(c, rhs(c, c->U, nullptr, nullptr, nullptr, c->fs, 0, 0, 0, 0));
+  HIPCHK(c, ensure_traces(c));
+  HIPCHK(c, rhs(c, c->U, c->TU, nullptr, nullptr, nullptr, nullptr, c->fs, 0, 0, 0, 0));
   HIPCHK(c, resid_out_dispatch(c->p, c->q, mesh_args(c), c->fs, c->Ua, R_host ? c->Ub : nullptr,
                                nullptr, c->stream));
   size_t sb = c->nstate * sizeof(double);
@@ -401,13 +443,19 @@ dgx_status dgx_residual(dgx_ctx* c, double* R_host, double* f_host)
 }
 
 // Launches one step; if want_norm, stores sum(R^2) of stage 1 in c->sum.
+// Every stage writes the trace records of its output state, so the next
+// stage (and the next step) reads them instead of recomputing.
 static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm)
 {
   hipError_t e;
+  if ((e = ensure_traces(c)) != hipSuccess)
+    return e;
+  c->tu_valid = false;  // until the last stage has rewritten them
   if (scheme == DGX_RK4)
   {
     // U1 = U + dt/2 f1 ; acc = f1
-    if ((e = rhs(c, c->U, c->U, c->acc, c->Ua, nullptr, 1.0, 0.0, 0.5 * dt, 1)) != hipSuccess)
+    if ((e = rhs(c, c->U, c->TU, c->U, c->acc, c->Ua, c->TUa, nullptr, 1.0, 0.0, 0.5 * dt, 1)) !=
+        hipSuccess)
       return e;
     if (want_norm)
     {
@@ -418,17 +466,23 @@ static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm)
         return e;
     }
     // U2 = U + dt/2 f2 ; acc += 2 f2
-    if ((e = rhs(c, c->Ua, c->U, c->acc, c->Ub, nullptr, 1.0, 0.0, 0.5 * dt, 2)) != hipSuccess)
+    if ((e = rhs(c, c->Ua, c->TUa, c->U, c->acc, c->Ub, c->TUb, nullptr, 1.0, 0.0, 0.5 * dt,
+                 2)) != hipSuccess)
       return e;
     // U3 = U + dt f3 ; acc += 2 f3
-    if ((e = rhs(c, c->Ub, c->U, c->acc, c->Ua, nullptr, 1.0, 0.0, dt, 2)) != hipSuccess)
+    if ((e = rhs(c, c->Ub, c->TUb, c->U, c->acc, c->Ua, c->TUa, nullptr, 1.0, 0.0, dt, 2)) !=
+        hipSuccess)
       return e;
     // U = U + dt/6 (acc + f4)
-    return rhs(c, c->Ua, c->U, c->acc, c->U, nullptr, 1.0, 0.0, dt, 3);
+    if ((e = rhs(c, c->Ua, c->TUa, c->U, c->acc, c->U, c->TU, nullptr, 1.0, 0.0, dt, 3)) !=
+        hipSuccess)
+      return e;
+    c->tu_valid = true;
+    return hipSuccess;
   }
   // TVD-RK3: U1 = U + dt f
-  if ((e = rhs(c, c->U, c->U, nullptr, c->Ua, want_norm ? c->fs : nullptr, 1.0, 0.0, dt, 0)) !=
-      hipSuccess)
+  if ((e = rhs(c, c->U, c->TU, c->U, nullptr, c->Ua, c->TUa, want_norm ? c->fs : nullptr, 1.0,
+               0.0, dt, 0)) != hipSuccess)
     return e;
   if (want_norm)
   {
@@ -439,10 +493,15 @@ static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm)
       return e;
   }
   // U2 = 3/4 U + 1/4 U1 + 1/4 dt f
-  if ((e = rhs(c, c->Ua, c->U, nullptr, c->Ub, nullptr, 0.75, 0.25, 0.25 * dt, 0)) != hipSuccess)
+  if ((e = rhs(c, c->Ua, c->TUa, c->U, nullptr, c->Ub, c->TUb, nullptr, 0.75, 0.25, 0.25 * dt,
+               0)) != hipSuccess)
     return e;
   // U = 1/3 U + 2/3 U2 + 2/3 dt f
-  return rhs(c, c->Ub, c->U, nullptr, c->U, nullptr, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt, 0);
+  if ((e = rhs(c, c->Ub, c->TUb, c->U, nullptr, c->U, c->TU, nullptr, 1.0 / 3.0, 2.0 / 3.0,
+               (2.0 / 3.0) * dt, 0)) != hipSuccess)
+    return e;
+  c->tu_valid = true;
+  return hipSuccess;
 }
 
 dgx_status dgx_device_copy(dgx_ctx* c, void* dst, const void* src, uint64_t bytes)
@@ -486,3 +545,71 @@ dgx_status dgx_step(dgx_ctx* c, int scheme, double dt, uint64_t nsteps, double* 
 }
 
 }  // extern "C"
+
+dgx_status dgx_set_tuning(dgx_ctx* c, int stage_waves)
+{
+  if (!c || stage_waves < 1 || stage_waves > 3)
+    return DGX_ERR_INVALID;
+  c->stage_waves = stage_waves;
+  return DGX_OK;
+}
+
+dgx_status dgx_get_tuning(const dgx_ctx* c, int* stage_waves)
+{
+  if (!c || !stage_waves)
+    return DGX_ERR_INVALID;
+  *stage_waves = c->stage_waves;
+  return DGX_OK;
+}
+
+// Times the stage kernel at each occupancy target on the context's own mesh:
+// U (with its traces) -> Ub, TUb, which are stage scratch between steps.
+dgx_status dgx_autotune(dgx_ctx* c, int reps, int* chosen, double us_out[3])
+{
+  if (!c || reps < 1)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, ensure_traces(c));
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  int keep = c->stage_waves, best = keep;
+  double best_us = 0.0;
+  dgx_status st = DGX_OK;
+  for (int w = 1; w <= 3 && st == DGX_OK; ++w)
+  {
+    c->stage_waves = w;
+    hipError_t e = rhs(c, c->U, c->TU, c->U, nullptr, c->Ub, c->TUb, nullptr, 1.0, 0.0, 0.0, 0);
+    if (e == hipSuccess)
+      e = hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r)
+      e = rhs(c, c->U, c->TU, c->U, nullptr, c->Ub, c->TUb, nullptr, 1.0, 0.0, 0.0, 0);
+    if (e == hipSuccess)
+      e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess)
+      e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess)
+      e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess)
+    {
+      st = fail(c, DGX_ERR_HIP, "autotune: %s", hipGetErrorString(e));
+      break;
+    }
+    double us = 1e3 * (double)ms / reps;
+    if (us_out)
+      us_out[w - 1] = us;
+    if (w == 1 || us < best_us)
+    {
+      best_us = us;
+      best = w;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->stage_waves = (st == DGX_OK) ? best : keep;
+  if (chosen)
+    *chosen = c->stage_waves;
+  if (st != DGX_OK)
+    return st;
+  return check_status(c);
+}

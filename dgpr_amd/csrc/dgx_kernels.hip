@@ -19,11 +19,13 @@
 // 1/2 and with A at the boundary state on boundary faces (solve.cpp:642-689).
 //
 // One 64-lane wavefront processes 64/(p+1)^3 elements (one element at p=3),
-// one lane per Gauss node.  Each element evaluates its own six faces with the
-// neighbour's trace recomputed from the neighbour's state, so there are no
-// atomics and no face-residual buffers; every shared face is evaluated once
-// from each side (as the reference does on its processor-edge faces,
-// solve.cpp:400, 530, 607).
+// one lane per Gauss node.  Each element evaluates its own six faces, so
+// there are no atomics and no face-residual buffers; every shared face is
+// evaluated once from each side (as the reference does on its processor-edge
+// faces, solve.cpp:400, 530, 607).  The two traces of a face point (state
+// and gradient traction) come from trace records that the previous stage
+// wrote together with its output state, so no element recomputes a
+// neighbour's gradient.
 #include <hip/hip_runtime.h>
 
 #include "dgx_device.hpp"
@@ -118,11 +120,9 @@ __device__ __forceinline__ void inv3(const double J[3][3], double det, double Ji
 
 // Geometry-basis 1D values for the three directions of a volume node (i,j,k)
 // or a face point (end e on direction nd, Gauss indices elsewhere).
-template <int P, int Q>
-__device__ __forceinline__ void geo_vals(int nd, int e, int i, int j, int k, const double** v,
-                                         const double** d)
+__device__ __forceinline__ void geo_vals(const QTab& qt, int nd, int e, int i, int j, int k,
+                                         const double** v, const double** d)
 {
-  const QTab& qt = c_qtab[P - 1][Q - 1];
   int idx[3] = {i, j, k};
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -140,14 +140,37 @@ __device__ __forceinline__ void geo_vals(int nd, int e, int i, int j, int k, con
   }
 }
 
-template <int P, int Q>
-__device__ __forceinline__ void jac_at(const double* X, int nd, int e, int i, int j, int k,
-                                       double J[3][3])
+template <int Q>
+__device__ __forceinline__ void jac_at(const QTab& qt, const double* X, int nd, int e, int i,
+                                       int j, int k, double J[3][3])
 {
   const double* v[3];
   const double* d[3];
-  geo_vals<P, Q>(nd, e, i, j, k, v, d);
+  geo_vals(qt, nd, e, i, j, k, v, d);
   jacobian<Q>(X, v[0], d[0], v[1], d[1], v[2], d[2], J);
+}
+
+// Copies the 1D tables of (P, Q) into LDS: the lane-dependent table reads of
+// the stage kernels then become LDS reads instead of vector loads.
+template <int P, int Q>
+__device__ __forceinline__ void stage_tables(PTab* sPT, QTab* sQT)
+{
+  const double* a = reinterpret_cast<const double*>(&c_ptab[P - 1]);
+  double* b       = reinterpret_cast<double*>(sPT);
+  for (int x = threadIdx.x; x < (int)(sizeof(PTab) / 8); x += 64)
+    b[x] = a[x];
+  const double* c = reinterpret_cast<const double*>(&c_qtab[P - 1][Q - 1]);
+  double* d       = reinterpret_cast<double*>(sQT);
+  for (int x = threadIdx.x; x < (int)(sizeof(QTab) / 8); x += 64)
+    d[x] = c[x];
+}
+
+// k-th of three values, selected by value: a conditional between array
+// elements would select between their addresses and keep the array in
+// scratch memory.
+__device__ __forceinline__ double pick3(int k, double a, double b, double c)
+{
+  return (k == 0) ? a : ((k == 1) ? b : c);
 }
 
 // Outward unit normal of local face (nd, hi) from the Jacobian at a face
@@ -155,17 +178,17 @@ __device__ __forceinline__ void jac_at(const double* X, int nd, int e, int i, in
 // on low faces (geometry_common.cpp:416-454).  Returns the surface element.
 __device__ __forceinline__ double face_normal(const double J[3][3], int nd, int hi, double* nv)
 {
-  double u0 = (nd == 0) ? J[0][1] : ((nd == 1) ? J[0][2] : J[0][0]);
-  double u1 = (nd == 0) ? J[1][1] : ((nd == 1) ? J[1][2] : J[1][0]);
-  double u2 = (nd == 0) ? J[2][1] : ((nd == 1) ? J[2][2] : J[2][0]);
-  double v0 = (nd == 0) ? J[0][2] : ((nd == 1) ? J[0][0] : J[0][1]);
-  double v1 = (nd == 0) ? J[1][2] : ((nd == 1) ? J[1][0] : J[1][1]);
-  double v2 = (nd == 0) ? J[2][2] : ((nd == 1) ? J[2][0] : J[2][1]);
+  double u0 = pick3(nd, J[0][1], J[0][2], J[0][0]);
+  double u1 = pick3(nd, J[1][1], J[1][2], J[1][0]);
+  double u2 = pick3(nd, J[2][1], J[2][2], J[2][0]);
+  double v0 = pick3(nd, J[0][2], J[0][0], J[0][1]);
+  double v1 = pick3(nd, J[1][2], J[1][0], J[1][1]);
+  double v2 = pick3(nd, J[2][2], J[2][0], J[2][1]);
   nv[0] = u1 * v2 - u2 * v1;
   nv[1] = u2 * v0 - u0 * v2;
   nv[2] = u0 * v1 - u1 * v0;
   double fJ = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-  double sg = (hi ? 1.0 : -1.0) / fJ;
+  double sg = (hi ? 1.0 : -1.0) * rcp(fJ);
   nv[0] *= sg;
   nv[1] *= sg;
   nv[2] *= sg;
@@ -177,22 +200,28 @@ __device__ __forceinline__ double face_normal(const double J[3][3], int nd, int 
 // non-periodic domain boundary (geometry_shared.cpp:199-229).
 __device__ __forceinline__ int neighbour(const MeshArgs& m, int i, int j, int k, int f, int* bt)
 {
-  int nd = f >> 1, hi = f & 1;
-  int c[3] = {i, j, k};
-  int n[3] = {m.nx, m.ny, m.nz};
-  int x = c[nd] + (hi ? 1 : -1);
-  if (x < 0 || x >= n[nd])
+  const int nd = f >> 1, hi = f & 1;
+  const int nx = m.nx, ny = m.ny, nz = m.nz;
+  const int c  = (nd == 0) ? i : ((nd == 1) ? j : k);
+  const int n  = (nd == 0) ? nx : ((nd == 1) ? ny : nz);
+  const int stride = (nd == 0) ? 1 : ((nd == 1) ? nx : nx * ny);
+  int x = c + (hi ? 1 : -1);
+  if (x < 0 || x >= n)
   {
-    if (m.bounds[f] != -2)
+    // boundary code of side f, selected by value (no dynamic indexing)
+    const int b01 = hi ? m.bounds[1] : m.bounds[0];
+    const int b23 = hi ? m.bounds[3] : m.bounds[2];
+    const int b45 = hi ? m.bounds[5] : m.bounds[4];
+    const int bc  = (nd == 0) ? b01 : ((nd == 1) ? b23 : b45);
+    if (bc != -2)
     {
-      *bt = m.bounds[f];
+      *bt = bc;
       return -1;
     }
-    x = (x + n[nd]) % n[nd];
+    x = hi ? 0 : n - 1;  // periodic wrap
   }
-  c[nd] = x;
   *bt = -1;
-  return (c[2] * m.ny + c[1]) * m.nx + c[0];
+  return ((k * ny + j) * nx + i) + (x - c) * stride;
 }
 
 // Faces are processed in passes of whole faces, one face point per lane:
@@ -274,7 +303,7 @@ __device__ __forceinline__ void grad_traction(const double gr[3][4], const doubl
                                               const double Ji[3][3], const double* n, double mu,
                                               double* vg)
 {
-  double ir = 1.0 / U[0];
+  double ir = rcp(U[0]);
   double u[3] = {U[1] * ir, U[2] * ir, U[3] * ir};
   double Ln[3] = {0.0, 0.0, 0.0}, LTn[3] = {0.0, 0.0, 0.0}, tr = 0.0;
 #pragma unroll
@@ -306,13 +335,13 @@ __device__ __forceinline__ void face_point_physics(bool interior, const double* 
                                                    FacePoint& out)
 {
   const double mu = ph.mu;
-  double irL = 1.0 / UL[0];
+  double irL = rcp(UL[0]);
   double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
   if (interior)
     roe_flux(UL, Ub, n, ph.gamma, out.G);
   else
     euler_flux_n(Ub, ph.gamma, n, out.G);
-  double irB = 1.0 / Ub[0];
+  double irB = rcp(Ub[0]);
   double uB[3] = {Ub[1] * irB, Ub[2] * irB, Ub[3] * irB};
   double jr = UL[0] - Ub[0];
   double dmL[3], dmB[3];
@@ -363,26 +392,160 @@ __device__ __forceinline__ void face_point_physics(bool interior, const double* 
   out.S[5] = mu * (a1 * n[2] + a2 * n[1]);
 }
 
-// One full right-hand-side evaluation f_g = B3 f of EPW elements fused with
-// the RK stage combination of StageArgs.  lam[e][f][pt] holds the BR2
-// lifting coefficient of element e's side of its face f.
-#ifndef DGX_STAGE_WAVES_PER_EU
-#define DGX_STAGE_WAVES_PER_EU 2
-#endif
+// Face trace records of the EPW elements whose Gauss-nodal states are in sV:
+// for each local face f and face point, the trace U (5) and the gradient
+// traction tau(grad u).n (3) taken with the element's own outward unit
+// normal n, stored as Tr[e][f][8][(p+1)^2].  The face point (a, b) indexes
+// the Gauss points of the two tangential directions in increasing order, so
+// the point of face f of e and of face f^1 of its neighbour coincide
+// (basis.cpp:180-192).  Called by every thread of the block; sR needs
+// 4*64 doubles.
 template <int P, int Q>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DGX_STAGE_WAVES_PER_EU)))
+__device__ __forceinline__ void produce_traces(const MeshArgs& mesh, double mu, const PTab& pt,
+                                               const QTab& qt,
+                                               const double (*sV)[5][FaceDims<P>::NB],
+                                               const double (*sX)[(Q + 1) * (Q + 1) * (Q + 1) * 3],
+                                               double* sR, int e0, double* __restrict__ Tout)
+{
+  using FD = FaceDims<P>;
+  constexpr int N = FD::N, NF = FD::NF;
+  
+  const int tid = threadIdx.x;
+  for (int pass = 0; pass < FD::NPASS; ++pass)
+  {
+    const int slot = tid / NF;
+    const int gf   = pass * FD::FPP + slot;
+    const int fpt  = tid % NF;
+    const int a = fpt % N, b = fpt / N;
+    const int l = gf / 6, f = gf % 6;
+    const int ee = e0 + l;
+    const bool fact = (slot < FD::FPP) && (gf < FD::NFACE) && (ee < mesh.nelem);
+    const int nd = f >> 1, hi = f & 1;
+    const int fbase = slot * NF;
+
+    double U[5], dn[4];
+    if (fact)
+    {
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+      {
+        double v = 0.0, dv = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+        {
+          double x = sV[l][r][line_node<N>(nd, m, a, b)];
+          v += pt.L[hi][m] * x;
+          if (r < 4)
+            dv += pt.Ld[hi][m] * x;
+        }
+        U[r] = v;
+        if (r < 4)
+        {
+          dn[r] = dv;
+          sR[tid * 4 + r] = v;
+        }
+      }
+    }
+    __syncthreads();
+    if (fact)
+    {
+      const int gi0 = (nd == 0) ? 0 : a;
+      const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
+      const int gi2 = (nd == 2) ? 0 : b;
+      double J[3][3], Ji[3][3];
+      jac_at<Q>(qt, sX[l], nd, hi, gi0, gi1, gi2, J);
+      inv3(J, det3(J), Ji);
+      double nv[3];
+      face_normal(J, nd, hi, nv);
+      double gr[3][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+        {
+          d1 += pt.Dg[a][m] * sR[(fbase + m + N * b) * 4 + r];
+          d2 += pt.Dg[b][m] * sR[(fbase + a + N * m) * 4 + r];
+        }
+        const double dr = dn[r];
+        gr[0][r] = pick3(nd, dr, d1, d1);
+        gr[1][r] = pick3(nd, d1, dr, d2);
+        gr[2][r] = pick3(nd, d2, d2, dr);
+      }
+      double vg[3];
+      grad_traction(gr, U, Ji, nv, mu, vg);
+      double* T = Tout + ((size_t)ee * 6 + f) * 8 * NF + fpt;
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+        T[r * NF] = U[r];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        T[(5 + i) * NF] = vg[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Trace records of a Gauss-nodal state V (the first stage after the state
+// was set from outside, and stand-alone residual evaluations).
+template <int P, int Q>
+__global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, double mu,
+                                                       const double* __restrict__ V,
+                                                       double* __restrict__ Tout)
+{
+  using FD = FaceDims<P>;
+  constexpr int NB = FD::NB, EPW = FD::EPW;
+  constexpr int NG = (Q + 1) * (Q + 1) * (Q + 1);
+  __shared__ double sV[EPW][5][NB];
+  __shared__ double sX[EPW][NG * 3];
+  __shared__ double sR[4 * 64];
+  __shared__ PTab sPT;
+  __shared__ QTab sQT;
+  stage_tables<P, Q>(&sPT, &sQT);
+  const int tid = threadIdx.x;
+  const int le  = tid / NB;
+  const int nd0 = tid % NB;
+  const int e0  = blockIdx.x * EPW;
+  if (le < EPW && e0 + le < mesh.nelem)
+  {
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+      sV[le][r][nd0] = V[((size_t)(e0 + le) * 5 + r) * NB + nd0];
+  }
+  for (int x = tid; x < EPW * NG * 3; x += 64)
+  {
+    int ee = e0 + x / (NG * 3);
+    sX[x / (NG * 3)][x % (NG * 3)] =
+        (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
+  }
+  __syncthreads();
+  produce_traces<P, Q>(mesh, mu, sPT, sQT, sV, sX, sR, e0, Tout);
+}
+
+// One full right-hand-side evaluation f_g = B3 f of EPW elements fused with
+// the RK stage combination of StageArgs, reading the face trace records Tin
+// of Vin and (optionally) writing those of Vout to Tout.  lam[e][f][pt]
+// holds the BR2 lifting coefficient of element e's side of its face f.
+// W is the occupancy target (waves per SIMD) handed to the register
+// allocator; the launch picks one of the compiled targets per (P, Q).
+template <int P, int Q, int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
                           int* status)
 {
   using FD = FaceDims<P>;
   constexpr int N = FD::N, NB = FD::NB, NF = FD::NF, EPW = FD::EPW;
   constexpr int NG = (Q + 1) * (Q + 1) * (Q + 1);
-  const PTab& pt = c_ptab[P - 1];
 
   __shared__ double sV[EPW][5][NB];
   __shared__ double sX[EPW][NG * 3];      // own geometry nodes
-  __shared__ double sXn[EPW][6][NG * 3];  // geometry nodes of the six neighbours
   __shared__ double sR[FD::RG];  // volume flux of one direction, then one face pass
+  __shared__ PTab sPT;
+  __shared__ QTab sQT;
+  stage_tables<P, Q>(&sPT, &sQT);
+  const PTab& pt = sPT;
+  const QTab& qt = sQT;
 
   const int tid = threadIdx.x;
   const int le  = tid / NB;
@@ -404,22 +567,6 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
     int ee = e0 + l;
     sX[l][x % (NG * 3)] = (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
   }
-  for (int x = tid; x < EPW * 6 * NG * 3; x += 64)
-  {
-    int k  = x % (NG * 3);
-    int fl = x / (NG * 3);
-    int ee = e0 + fl / 6;
-    double v = 0.0;
-    if (ee < mesh.nelem)
-    {
-      int bt;
-      int en = neighbour(mesh, ee % mesh.nx, (ee / mesh.nx) % mesh.ny, ee / (mesh.nx * mesh.ny),
-                         fl % 6, &bt);
-      if (en >= 0)
-        v = mesh.nodes[(size_t)en * NG * 3 + k];
-    }
-    sXn[fl / 6][fl % 6][k] = v;
-  }
   __syncthreads();
 
   const int idx[3] = {nd0 % N, (nd0 / N) % N, nd0 / (N * N)};
@@ -432,7 +579,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
   if (vact)
   {
     double J[3][3];
-    jac_at<P, Q>(sX[le], -1, 0, idx[0], idx[1], idx[2], J);
+    jac_at<Q>(qt, sX[le], -1, 0, idx[0], idx[1], idx[2], J);
     double det = det3(J);
     inv3(J, det, Ji);
     wJ = pt.w[idx[0]] * pt.w[idx[1]] * pt.w[idx[2]] * det;
@@ -455,7 +602,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
       gr[1][r] = g1;
       gr[2][r] = g2;
     }
-    double ir = 1.0 / Vq[0];
+    double ir = rcp(Vq[0]);
     double u[3] = {Vq[1] * ir, Vq[2] * ir, Vq[3] * ir};
     double Lv[3][3];
     vel_grad(gr, u, ir, Ji, Lv);
@@ -509,9 +656,11 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
   }
 
   // ---- faces, in passes of FPP whole faces ----
-  // per face point in sR (stride 20): pass A: UL (0-4), UR (5-9);
-  // pass B: G (0-4), normal contravariant dcQ (5-9), tangential (10-19);
-  // pass C: T = G - Dg^T H_t1 - Dg^T H_t2 replaces G.
+  // Both traces of a face point come from the trace records Tin written with
+  // the input state (own face f, and the neighbour's face f^1 at the same
+  // face point).  Per face point in sR (stride 20): G (0-4), normal
+  // contravariant dcQ (5-8), tangential (10-13, 15-18); the T phase then
+  // replaces G by T = G - Dg^T H_t1 - Dg^T H_t2.
   bool ok = true;
   for (int pass = 0; pass < FD::NPASS; ++pass)
   {
@@ -525,144 +674,57 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
     const int nd = f >> 1, hi = f & 1;
     const int fbase = slot * NF;
 
-    double UL[5], UR[5], dnL[4], dnR[4];
-    int en = -1, bt = -1;
-    if (fact)
-    {
-      int ci = ee % mesh.nx, cj = (ee / mesh.nx) % mesh.ny, ck = ee / (mesh.nx * mesh.ny);
-      en = neighbour(mesh, ci, cj, ck, f, &bt);
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-      {
-        double v = 0.0, dv = 0.0;
-#pragma unroll
-        for (int m = 0; m < N; ++m)
-        {
-          double x = sV[l][r][line_node<N>(nd, m, a, b)];
-          v += pt.L[hi][m] * x;
-          if (r < 4)
-            dv += pt.Ld[hi][m] * x;
-        }
-        UL[r] = v;
-        if (r < 4)
-          dnL[r] = dv;
-      }
-      if (en >= 0)
-      {
-        const double* Vn = sa.Vin + (size_t)en * 5 * NB;
-#pragma unroll
-        for (int r = 0; r < 5; ++r)
-        {
-          double v = 0.0, dv = 0.0;
-#pragma unroll
-          for (int m = 0; m < N; ++m)
-          {
-            double x = Vn[r * NB + line_node<N>(nd, m, a, b)];
-            v += pt.L[1 - hi][m] * x;
-            if (r < 4)
-              dv += pt.Ld[1 - hi][m] * x;
-          }
-          UR[r] = v;
-          if (r < 4)
-            dnR[r] = dv;
-        }
-      }
-      else
-      {
-#pragma unroll
-        for (int r = 0; r < 5; ++r)
-          UR[r] = 0.0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          dnR[r] = 0.0;
-      }
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-      {
-        sR[tid * 20 + r]     = UL[r];
-        sR[tid * 20 + 5 + r] = UR[r];
-      }
-    }
-    __syncthreads();
-
     FacePoint fpo;
     double hn[4], h1[4], h2[4];
     if (fact)
     {
+      int bt = -1;
+      const int en = neighbour(mesh, ee % mesh.nx, (ee / mesh.nx) % mesh.ny,
+                               ee / (mesh.nx * mesh.ny), f, &bt);
+      const double* To = sa.Tin + ((size_t)ee * 6 + f) * 8 * NF + fpt;
+      double UL[5], vgL[3];
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+        UL[r] = To[r * NF];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        vgL[i] = To[(5 + i) * NF];
+
       // own geometry at the face point (end hi on direction nd)
       const int gi0 = (nd == 0) ? 0 : a;
       const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
       const int gi2 = (nd == 2) ? 0 : b;
       double J[3][3], JiL[3][3];
-      jac_at<P, Q>(sX[l], nd, hi, gi0, gi1, gi2, J);
+      jac_at<Q>(qt, sX[l], nd, hi, gi0, gi1, gi2, J);
       inv3(J, det3(J), JiL);
       double nv[3];
-      double fJ = face_normal(J, nd, hi, nv);
-      double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
+      const double fJ = face_normal(J, nd, hi, nv);
+      const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
 
-      // neighbour side first, reduced to its gradient traction
+      // neighbour trace; its traction was taken with its own outward normal -n
+      double Ub[5], vgR[3] = {0.0, 0.0, 0.0};
       double lamR = 0.0;
-      double vgR[3] = {0.0, 0.0, 0.0};
       if (en >= 0)
       {
-        double JR[3][3], JiR[3][3];
-        jac_at<P, Q>(sXn[l][f], nd, 1 - hi, gi0, gi1, gi2, JR);
-        inv3(JR, det3(JR), JiR);
-        lamR = lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
-        double grR[3][4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-        {
-          double e1 = 0.0, e2 = 0.0;
-#pragma unroll
-          for (int m = 0; m < N; ++m)
-          {
-            e1 += pt.Dg[a][m] * sR[(fbase + m + N * b) * 20 + 5 + r];
-            e2 += pt.Dg[b][m] * sR[(fbase + a + N * m) * 20 + 5 + r];
-          }
-          grR[0][r] = (nd == 0) ? dnR[r] : e1;
-          grR[1][r] = (nd == 1) ? dnR[r] : ((nd == 0) ? e1 : e2);
-          grR[2][r] = (nd == 2) ? dnR[r] : e2;
-        }
-        grad_traction(grR, UR, JiR, nv, ph.mu, vgR);
-      }
-      // own side
-      double vgL[3];
-      {
-        double grL[3][4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-        {
-          double d1 = 0.0, d2 = 0.0;
-#pragma unroll
-          for (int m = 0; m < N; ++m)
-          {
-            d1 += pt.Dg[a][m] * sR[(fbase + m + N * b) * 20 + r];
-            d2 += pt.Dg[b][m] * sR[(fbase + a + N * m) * 20 + r];
-          }
-          grL[0][r] = (nd == 0) ? dnL[r] : d1;
-          grL[1][r] = (nd == 1) ? dnL[r] : ((nd == 0) ? d1 : d2);
-          grL[2][r] = (nd == 2) ? dnL[r] : d2;
-        }
-        grad_traction(grL, UL, JiL, nv, ph.mu, vgL);
-      }
-      double irL = 1.0 / UL[0];
-      double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
-      double Ub[5];
-      if (en >= 0)
-      {
+        const double* Tn = sa.Tin + ((size_t)en * 6 + (f ^ 1)) * 8 * NF + fpt;
 #pragma unroll
         for (int r = 0; r < 5; ++r)
-          Ub[r] = UR[r];
+          Ub[r] = Tn[r * NF];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          vgR[i] = -Tn[(5 + i) * NF];
+        lamR = lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
       }
       else
         ok = boundary_state(bt, UL, nv, ph, Ub) && ok;
       face_point_physics(en >= 0, UL, Ub, vgL, vgR, nv, lamL, lamR, ph, fpo);
 
       // contravariant dual-consistency flux h_c = Ji[c] . dcQ (rows 1-4)
-      double sc = fJ * pt.w[a] * pt.w[b];
+      const double irL = rcp(UL[0]);
+      const double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
+      const double sc = fJ * pt.w[a] * pt.w[b];
       const double* S = fpo.S;
-      double Sm[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
+      const double Sm[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
       double hc[3][4];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
@@ -675,20 +737,16 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
 #pragma unroll
       for (int r = 0; r < 4; ++r)
       {
-        hn[r] = (nd == 0) ? hc[0][r] : ((nd == 1) ? hc[1][r] : hc[2][r]);
-        h1[r] = (nd == 0) ? hc[1][r] : hc[0][r];
-        h2[r] = (nd == 2) ? hc[1][r] : hc[2][r];
+        hn[r] = pick3(nd, hc[0][r], hc[1][r], hc[2][r]);
+        h1[r] = pick3(nd, hc[1][r], hc[0][r], hc[0][r]);
+        h2[r] = pick3(nd, hc[2][r], hc[2][r], hc[1][r]);
       }
 #pragma unroll
       for (int r = 0; r < 5; ++r)
+      {
         fpo.G[r] *= sc;
-    }
-    __syncthreads();
-    if (fact)
-    {
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
         sR[tid * 20 + r] = fpo.G[r];
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r)
       {
@@ -745,7 +803,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
   // ---- form f_g and the RK epilogue ----
   if (vact)
   {
-    double iwJ = 1.0 / wJ;
+    double iwJ = rcp(wJ);
     size_t base = (size_t)e * 5 * NB + nd0;
 #pragma unroll
     for (int r = 0; r < 5; ++r)
@@ -769,8 +827,16 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
         if (sa.cb != 0.0)
           o += sa.cb * sV[le][r][nd0];
         sa.Vout[ix] = o;
+        sV[le][r][nd0] = o;  // only this lane reads this node from here on
       }
     }
+  }
+
+  // ---- face traces of the new state for the next stage ----
+  if (sa.Tout)
+  {
+    __syncthreads();
+    produce_traces<P, Q>(mesh, ph.mu, pt, qt, sV, sX, sR, e0, sa.Tout);
   }
 }
 
@@ -799,7 +865,7 @@ __global__ __launch_bounds__(64) void dgx_lift_coef_kernel(MeshArgs mesh, double
   gi[t1] = a;
   gi[t2] = b;
   double J[3][3];
-  jac_at<P, Q>(X, nd, hi, gi[0], gi[1], gi[2], J);
+  jac_at<Q>(c_qtab[P - 1][Q - 1], X, nd, hi, gi[0], gi[1], gi[2], J);
   int c1 = (nd + 1) % 3, c2 = (nd + 2) % 3;
   double n0 = J[1][c1] * J[2][c2] - J[2][c1] * J[1][c2];
   double n1 = J[2][c1] * J[0][c2] - J[0][c1] * J[2][c2];
@@ -813,7 +879,7 @@ __global__ __launch_bounds__(64) void dgx_lift_coef_kernel(MeshArgs mesh, double
     gm[t1] = a;
     gm[t2] = b;
     double Jm[3][3];
-    jac_at<P, Q>(X, -1, 0, gm[0], gm[1], gm[2], Jm);
+    jac_at<Q>(c_qtab[P - 1][Q - 1], X, -1, 0, gm[0], gm[1], gm[2], Jm);
     s += pt.L[hi][m] * pt.L[hi][m] / (pt.w[m] * det3(Jm));
   }
   lam[t] = fJ * s;
@@ -1004,7 +1070,7 @@ __global__ __launch_bounds__(64) void dgx_resid_out_kernel(MeshArgs mesh, const 
     if (act)
     {
       double J[3][3];
-      jac_at<P, Q>(sX[le], -1, 0, ni, nj, nk, J);
+      jac_at<Q>(c_qtab[P - 1][Q - 1], sX[le], -1, 0, ni, nj, nk, J);
       double wJ = pt.w[ni] * pt.w[nj] * pt.w[nk] * det3(J);
 #pragma unroll
       for (int r = 0; r < 5; ++r)
@@ -1083,12 +1149,31 @@ hipError_t copy_dispatch(const void* src, void* dst, size_t bytes, hipStream_t s
 
 template <int P, int Q>
 hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
-                      int* status, hipStream_t st)
+                      int* status, int waves, hipStream_t st)
 {
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
-  hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, sa, lam,
-                     status);
+  if (waves == 1)
+    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 1>), dim3(nblk), dim3(64), 0, st, m, ph, sa,
+                       lam, status);
+  else if (waves == 2)
+    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 2>), dim3(nblk), dim3(64), 0, st, m, ph, sa,
+                       lam, status);
+  else if (waves == 3)
+    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 3>), dim3(nblk), dim3(64), 0, st, m, ph, sa,
+                       lam, status);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int P, int Q>
+hipError_t launch_trace(const MeshArgs& m, double mu, const double* V, double* Tout,
+                        hipStream_t st)
+{
+  constexpr int EPW = Dims<P, Q>::EPW;
+  int nblk = (m.nelem + EPW - 1) / EPW;
+  hipLaunchKernelGGL((dgx_trace_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, mu, V, Tout);
   return hipGetLastError();
 }
 
@@ -1144,11 +1229,19 @@ int resid_out_blocks(int p, int nelem)
   return hipErrorInvalidValue;
 
 hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
-                        const double* lam, int* status, hipStream_t st)
+                        const double* lam, int* status, int waves, hipStream_t st)
 {
-#define CALL_RHS(P, Q) launch_rhs<P, Q>(m, ph, sa, lam, status, st)
+#define CALL_RHS(P, Q) launch_rhs<P, Q>(m, ph, sa, lam, status, waves, st)
   DGX_DISPATCH_PQ(p, q, CALL_RHS)
 #undef CALL_RHS
+}
+
+hipError_t trace_dispatch(int p, int q, const MeshArgs& m, double mu, const double* V,
+                          double* Tout, hipStream_t st)
+{
+#define CALL_TR(P, Q) launch_trace<P, Q>(m, mu, V, Tout, st)
+  DGX_DISPATCH_PQ(p, q, CALL_TR)
+#undef CALL_TR
 }
 
 hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipStream_t st)

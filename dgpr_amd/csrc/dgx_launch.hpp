@@ -56,7 +56,10 @@ struct MeshArgs
 //   Vout = ca*Vbase + cb*Vin + cf*fterm
 // with fterm = f for accmode 0..2 and (acc + f)/6 for accmode 3;
 // accmode 1 stores acc = f, accmode 2 updates acc += 2 f.  fout (optional)
-// receives f.  All states are Gauss-nodal [e][5][(p+1)^3].
+// receives f.  All states are Gauss-nodal [e][5][(p+1)^3].  Tin holds the
+// face trace records of Vin; Tout (optional, only with Vout) receives those
+// of Vout.  A trace record set is [e][6 faces][8][(p+1)^2]: the trace (5)
+// and the gradient traction tau(grad u).n with the own outward normal (3).
 struct StageArgs
 {
   const double* Vin;
@@ -64,12 +67,19 @@ struct StageArgs
   double* acc;
   double* Vout;
   double* fout;
+  const double* Tin;
+  double* Tout;
   double ca, cb, cf;
   int accmode;
 };
 
+// waves: occupancy target of the compiled stage kernel (1, 2 or 3 waves per
+// SIMD), chosen by default_stage_waves or by timing (dgx_autotune)
 hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
-                        const double* lam, int* status, hipStream_t st);
+                        const double* lam, int* status, int waves, hipStream_t st);
+// face trace records Tout of the Gauss-nodal state V
+hipError_t trace_dispatch(int p, int q, const MeshArgs& m, double mu, const double* V,
+                          double* Tout, hipStream_t st);
 // lam[e][f][(p+1)^2]: BR2 lifting coefficient of each element side of each face
 hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipStream_t st);
 hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg, double* f,

@@ -105,12 +105,16 @@ def lib():
         L.dgx_write_dg.argtypes = [C.c_char_p, U64, C.POINTER(_CMesh), PD]
         L.dgx_read_dg_header.argtypes = [C.c_char_p, C.POINTER(U64), C.POINTER(_CMesh)]
         L.dgx_read_dg.argtypes = [C.c_char_p, PD, PD]
+        L.dgx_set_tuning.argtypes = [P_ctx, C.c_int]
+        L.dgx_get_tuning.argtypes = [P_ctx, C.POINTER(C.c_int)]
+        L.dgx_autotune.argtypes = [P_ctx, C.c_int, C.POINTER(C.c_int), PD]
         for nm in ("dgx_create", "dgx_destroy", "dgx_set_params", "dgx_set_state",
                    "dgx_get_state", "dgx_set_state_device", "dgx_get_state_device",
                    "dgx_residual", "dgx_residual_device", "dgx_step", "dgx_step_async", "dgx_device_copy",
                    "dgx_sync", "dgx_get_stream", "dgx_get_sizes", "dgx_make_reference_state",
                    "dgx_bump_nodes", "dgx_box_nodes", "dgx_tgv_state", "dgx_channel_state",
-                   "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg"):
+                   "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg", "dgx_set_tuning",
+                   "dgx_get_tuning", "dgx_autotune"):
             getattr(L, nm).restype = st
         _lib = L
     return _lib
@@ -304,3 +308,22 @@ class Solver:
         s = C.c_void_p()
         _chk(lib().dgx_get_stream(self.ctx, C.byref(s)), self.ctx)
         return s.value
+
+    @property
+    def stage_waves(self):
+        """Occupancy target (waves per SIMD) of the stage kernel."""
+        w = C.c_int()
+        _chk(lib().dgx_get_tuning(self.ctx, C.byref(w)), self.ctx)
+        return w.value
+
+    @stage_waves.setter
+    def stage_waves(self, w):
+        _chk(lib().dgx_set_tuning(self.ctx, int(w)), self.ctx)
+
+    def autotune(self, reps=5):
+        """Times each compiled occupancy target on this mesh, keeps the
+        fastest; returns (chosen, [us at 1, 2, 3 waves/SIMD])."""
+        w = C.c_int()
+        us = np.zeros(3)
+        _chk(lib().dgx_autotune(self.ctx, int(reps), C.byref(w), _p(us)), self.ctx)
+        return w.value, us

@@ -171,6 +171,21 @@ dgx_status dgx_get_sizes(const dgx_ctx* ctx, uint64_t out[4]);
  * failed dgx_create when ctx is NULL). */
 const char* dgx_last_error(const dgx_ctx* ctx);
 
+/* ---- launch tuning (no counterpart in the reference, whose kernels have
+ * fixed launch shapes, solve_cuda.cu:1235-1282) ---------------------------- */
+
+/* The stage kernel is compiled for occupancy targets of 1, 2 and 3 waves per
+ * SIMD (register budgets of 512, 256 and 168 VGPRs).  dgx_create picks a
+ * default per (p, q); these set or read the target used by later stages. */
+dgx_status dgx_set_tuning(dgx_ctx* ctx, int stage_waves);
+dgx_status dgx_get_tuning(const dgx_ctx* ctx, int* stage_waves);
+
+/* Times every compiled occupancy target on this context's mesh and state
+ * (reps launches each into scratch buffers; the state is unchanged), keeps
+ * the fastest and returns it in *chosen (may be NULL).  us_out (may be NULL)
+ * receives the three average launch times in microseconds. */
+dgx_status dgx_autotune(dgx_ctx* ctx, int reps, int* chosen, double us_out[3]);
+
 /* ---- host-side helpers (no device work) --------------------------------- */
 
 /* make_reference_state (solver_params.cpp:40-70): P = c^2 rho/gamma,

@@ -116,3 +116,59 @@ def test_solution_parity(name, orc, built):
     print("  rel err U %.3e  norms %.3e" % (eU, en), flush=True)
     assert eU < SOL_TOL
     assert en < SOL_TOL
+
+
+def test_trace_records_follow_state_and_params(orc, built):
+    """The device keeps face trace records of its state between calls; a
+    residual after steps, after set_state and after set_params must equal a
+    fresh evaluation of the same state and parameters (oracle)."""
+    case = CASES["channel_p2"]()
+    m = case.mesh
+    geo = orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
+    prm2 = case.params.copy()
+    prm2.mu = 3.0 * case.params.mu
+    with dgx.Solver(m, case.params) as s:
+        s.set_state(case.U0)
+        s.step(case.dt, case.scheme, 2)
+        U1 = s.get_state()
+        _, f1 = s.residual()
+        _, f1o = geo.residual(orc_params(orc, case.params), U1)
+        print("  after steps: rel err f %.3e" % rel(f1, f1o), flush=True)
+        assert rel(f1, f1o) < RES_TOL
+        s.set_params(prm2)
+        _, f2 = s.residual()
+        _, f2o = geo.residual(orc_params(orc, prm2), U1)
+        print("  after set_params: rel err f %.3e" % rel(f2, f2o), flush=True)
+        assert rel(f2, f2o) < RES_TOL
+        s.set_state(case.U0)
+        s.step(case.dt, case.scheme, 1)
+        U2 = s.get_state()
+        U2o, _ = geo.step(orc_params(orc, prm2), case.U0, case.dt, case.scheme, 1)
+        print("  step after set_state: rel err U %.3e" % rel(U2, U2o), flush=True)
+        assert rel(U2, U2o) < SOL_TOL
+
+
+def test_occupancy_targets_agree(orc, built):
+    """Every compiled occupancy target of the stage kernel gives the oracle's
+    residual; the autotuner keeps one of them."""
+    case = CASES["channel_p3"]()
+    m = case.mesh
+    geo = orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
+    _, f_o = geo.residual(orc_params(orc, case.params), case.U0)
+    with dgx.Solver(m, case.params) as s:
+        s.set_state(case.U0)
+        fs = []
+        for w in (1, 2, 3):
+            s.stage_waves = w
+            assert s.stage_waves == w
+            _, f = s.residual()
+            print("  waves %d: rel err f %.3e" % (w, rel(f, f_o)), flush=True)
+            assert rel(f, f_o) < RES_TOL
+            fs.append(f)
+        assert rel(fs[0], fs[2]) < 1e-14 and rel(fs[1], fs[2]) < 1e-14
+        chosen, us = s.autotune(reps=2)
+        print("  autotune chose %d, us %s" % (chosen, us), flush=True)
+        assert chosen in (1, 2, 3) and s.stage_waves == chosen
+        assert np.all(us > 0)
+        U_before = s.get_state()
+        assert rel(U_before, case.U0) < 1e-14  # tuning launches leave the state alone
