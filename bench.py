@@ -155,10 +155,16 @@ def main():
     finite = bool(dgx.np.isfinite(U).all())
     s.close()
 
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % a.workload)
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+    # HBM bytes and executed FP64 flops per stage-kernel launch, from the
+    # committed rocprofv3 counter summary of the same kernel and workload
+    kname = "dgx_rhs_stage_kernel<%d,%d,%d>" % (case.mesh.p, case.mesh.q, waves)
+    traffic = flops = None
+    cf = os.path.join(ROOT, "profiles", "counters_%s.json" % a.workload)
+    if os.path.exists(cf):
+        cnt = json.load(open(cf))
+        if cnt.get("kernel") == kname:
+            traffic = cnt.get("hbm_bytes_per_launch")
+            flops = cnt.get("fp64_flops_per_launch")
 
     out = {
         "metric": "GDOF-stage-updates/s, mixed vs FP64 GPU at stated error; adjoint/forward cost",
@@ -190,7 +196,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "dgx_rhs_stage_kernel<%d,%d,%d>" % (case.mesh.p, case.mesh.q, waves),
+            "kernel": kname,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -198,6 +204,16 @@ def main():
             "traffic": traffic,
             "alg_bytes_per_dof_stage": bpd,
             "avg_launch_us": avg_launch_s * 1e6,
+        },
+        # FP64 utilisation of the same kernel: executed flops (rocprofv3
+        # FP64 VALU counters) over the live average launch time
+        "fp64": None if flops is None else {
+            "achieved": flops / avg_launch_s / 1e12,
+            "peak": FP64_PEAK_TFS,
+            "unit": "TFLOP/s",
+            "frac": flops / avg_launch_s / 1e12 / FP64_PEAK_TFS,
+            "flops_per_launch": flops,
+            "flops_per_dof_stage": flops / case.dof,
         },
     }
     if not a.no_cpu_baseline:

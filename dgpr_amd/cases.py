@@ -90,5 +90,11 @@ def config1():
 
 
 def config2():
-    """64^3 periodic p=3 Taylor-Green, RK4 dt=5e-4, 100 steps."""
-    return tgv_case(n=64, p=3, seed=2, dt=5e-4, nsteps=100, scheme="RK4")
+    """64^3 periodic p=3 Taylor-Green, RK4, 100 steps at dt = 2.5e-4.
+
+    The nominal dt = 5e-4 exceeds the explicit stability limit of this
+    scheme at this resolution: the same CFL (dt c / h) on the 4^3 mesh
+    (dt = 8e-3) diverges after about 40 RK4 steps in the CPU restatement of
+    the reference residual as well as on the device, while dt c / h halved
+    stays bounded over 220 steps."""
+    return tgv_case(n=64, p=3, seed=2, dt=2.5e-4, nsteps=100, scheme="RK4")

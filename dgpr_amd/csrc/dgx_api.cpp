@@ -246,7 +246,7 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
     return bail("hipMalloc(lam)", e);
   if ((e = hipMalloc(&c->status, sizeof(int))) != hipSuccess)
     return bail("hipMalloc(status)", e);
-  size_t tb = nlam * 8 * sizeof(double);
+  size_t tb = trace_doubles(c->p, c->nx, c->ny, c->nz) * sizeof(double);
   double** tbufs[3] = {&c->TU, &c->TUa, &c->TUb};
   for (int i = 0; i < 3; ++i)
     if ((e = hipMalloc(tbufs[i], tb)) != hipSuccess)
@@ -396,7 +396,8 @@ static hipError_t ensure_traces(dgx_ctx* c)
 {
   if (c->tu_valid)
     return hipSuccess;
-  hipError_t e = trace_dispatch(c->p, c->q, mesh_args(c), c->phys.mu, c->U, c->TU, c->stream);
+  hipError_t e =
+      trace_dispatch(c->p, c->q, mesh_args(c), c->phys, c->U, c->TU, c->status, c->stream);
   if (e == hipSuccess)
     c->tu_valid = true;
   return e;
@@ -420,7 +421,8 @@ dgx_status dgx_residual_device(dgx_ctx* c, const double* U_dev, double* R_dev, d
   if (!c || !U_dev || !f_dev)
     return DGX_ERR_INVALID;
   HIPCHK(c, basis_dispatch(c->p, U_dev, c->Ua, c->nelem, 0, c->stream));
-  HIPCHK(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys.mu, c->Ua, c->TUa, c->stream));
+  HIPCHK(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys, c->Ua, c->TUa, c->status,
+                           c->stream));
   HIPCHK(c, rhs(c, c->Ua, c->TUa, nullptr, nullptr, nullptr, nullptr, c->fs, 0, 0, 0, 0));
   HIPCHK(c, resid_out_dispatch(c->p, c->q, mesh_args(c), c->fs, f_dev, R_dev, nullptr,
                                c->stream));

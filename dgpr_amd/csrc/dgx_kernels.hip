@@ -224,6 +224,49 @@ __device__ __forceinline__ int neighbour(const MeshArgs& m, int i, int j, int k,
   return ((k * ny + j) * nx + i) + (x - c) * stride;
 }
 
+// Index of the domain-boundary face on side f of element (i,j,k) among all
+// boundary faces: sides in order -x +x -y +y -z +z, each numbered by its two
+// tangential element coordinates.
+__device__ __forceinline__ int bface_index(const MeshArgs& m, int i, int j, int k, int f)
+{
+  const int nd = f >> 1, hi = f & 1;
+  const int sx = m.ny * m.nz, sy = m.nx * m.nz, sz = m.nx * m.ny;
+  const int base = (nd == 0) ? (hi ? sx : 0)
+                 : ((nd == 1) ? 2 * sx + (hi ? sy : 0) : 2 * sx + 2 * sy + (hi ? sz : 0));
+  const int in = (nd == 0) ? k * m.ny + j : ((nd == 1) ? k * m.nx + i : j * m.nx + i);
+  return base + in;
+}
+
+// Boundary faces of the whole mesh (all six sides).
+__host__ __device__ inline size_t boundary_faces(int nx, int ny, int nz)
+{
+  return 2 * ((size_t)ny * nz + (size_t)nx * nz + (size_t)nx * ny);
+}
+
+// Per-element face table built once per block: sNb[l][f] is the neighbour
+// across face f, or -1 - (boundary face index) on a domain boundary, whose
+// code goes to sBt[l][f].
+template <int EPW>
+__device__ __forceinline__ void face_table(const MeshArgs& mesh, int e0, int (*sNb)[6],
+                                           int (*sBt)[6])
+{
+  const int tid = threadIdx.x;
+  if (tid < EPW * 6)
+  {
+    const int l = tid / 6, f = tid % 6, ee = e0 + l;
+    int nb = 0, bt = -1;
+    if (ee < mesh.nelem)
+    {
+      const int ci = ee % mesh.nx, cj = (ee / mesh.nx) % mesh.ny, ck = ee / (mesh.nx * mesh.ny);
+      nb = neighbour(mesh, ci, cj, ck, f, &bt);
+      if (nb < 0)
+        nb = -1 - bface_index(mesh, ci, cj, ck, f);
+    }
+    sNb[l][f] = nb;
+    sBt[l][f] = bt;
+  }
+}
+
 // Faces are processed in passes of whole faces, one face point per lane:
 // FPP faces of NF points fit in one 64-lane pass.
 template <int P>
@@ -236,7 +279,8 @@ struct FaceDims
   static constexpr int FPP   = 64 / NF;
   static constexpr int NFACE = EPW * 6;
   static constexpr int NPASS = (NFACE + FPP - 1) / FPP;
-  static constexpr int RG    = (5 * NB * EPW > 20 * 64) ? 5 * NB * EPW : 20 * 64;
+  static constexpr int FS    = 17;  // doubles per face point in the face pass
+  static constexpr int RG    = (5 * NB * EPW > FS * 64) ? 5 * NB * EPW : FS * 64;
 };
 
 // ---- viscous flux in velocity-gradient form ---------------------------------
@@ -398,19 +442,24 @@ __device__ __forceinline__ void face_point_physics(bool interior, const double* 
 // normal n, stored as Tr[e][f][8][(p+1)^2].  The face point (a, b) indexes
 // the Gauss points of the two tangential directions in increasing order, so
 // the point of face f of e and of face f^1 of its neighbour coincide
-// (basis.cpp:180-192).  Called by every thread of the block; sR needs
-// 4*64 doubles.
+// (basis.cpp:180-192).  On a domain-boundary face the boundary state Ub
+// (bound_states.cpp:24-130) of the trace is stored as well, after the trace
+// records of all elements: Tb[boundary face][5][(p+1)^2].  Called by every
+// thread of the block; sR needs 4*64 doubles.
 template <int P, int Q>
-__device__ __forceinline__ void produce_traces(const MeshArgs& mesh, double mu, const PTab& pt,
-                                               const QTab& qt,
+__device__ __forceinline__ void produce_traces(const MeshArgs& mesh, const Phys& ph,
+                                               const PTab& pt, const QTab& qt,
                                                const double (*sV)[5][FaceDims<P>::NB],
                                                const double (*sX)[(Q + 1) * (Q + 1) * (Q + 1) * 3],
-                                               double* sR, int e0, double* __restrict__ Tout)
+                                               const int (*sNb)[6], const int (*sBt)[6],
+                                               double* sR, int e0, double* __restrict__ Tout,
+                                               int* status)
 {
   using FD = FaceDims<P>;
   constexpr int N = FD::N, NF = FD::NF;
-  
+  double* Tb = Tout + (size_t)mesh.nelem * 6 * 8 * NF;
   const int tid = threadIdx.x;
+  bool ok = true;
   for (int pass = 0; pass < FD::NPASS; ++pass)
   {
     const int slot = tid / NF;
@@ -474,7 +523,7 @@ __device__ __forceinline__ void produce_traces(const MeshArgs& mesh, double mu, 
         gr[2][r] = pick3(nd, d2, d2, dr);
       }
       double vg[3];
-      grad_traction(gr, U, Ji, nv, mu, vg);
+      grad_traction(gr, U, Ji, nv, ph.mu, vg);
       double* T = Tout + ((size_t)ee * 6 + f) * 8 * NF + fpt;
 #pragma unroll
       for (int r = 0; r < 5; ++r)
@@ -482,17 +531,29 @@ __device__ __forceinline__ void produce_traces(const MeshArgs& mesh, double mu, 
 #pragma unroll
       for (int i = 0; i < 3; ++i)
         T[(5 + i) * NF] = vg[i];
+      const int nb = sNb[l][f];
+      if (nb < 0)
+      {
+        double Ub[5];
+        ok = boundary_state(sBt[l][f], U, nv, ph, Ub) && ok;
+        double* B = Tb + (size_t)(-1 - nb) * 5 * NF + fpt;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          B[r * NF] = Ub[r];
+      }
     }
     __syncthreads();
   }
+  if (!ok)
+    atomicOr(status, 1);
 }
 
 // Trace records of a Gauss-nodal state V (the first stage after the state
 // was set from outside, and stand-alone residual evaluations).
 template <int P, int Q>
-__global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, double mu,
+__global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
                                                        const double* __restrict__ V,
-                                                       double* __restrict__ Tout)
+                                                       double* __restrict__ Tout, int* status)
 {
   using FD = FaceDims<P>;
   constexpr int NB = FD::NB, EPW = FD::EPW;
@@ -502,11 +563,13 @@ __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, double mu,
   __shared__ double sR[4 * 64];
   __shared__ PTab sPT;
   __shared__ QTab sQT;
+  __shared__ int sNb[EPW][6], sBt[EPW][6];
   stage_tables<P, Q>(&sPT, &sQT);
   const int tid = threadIdx.x;
   const int le  = tid / NB;
   const int nd0 = tid % NB;
   const int e0  = blockIdx.x * EPW;
+  face_table<EPW>(mesh, e0, sNb, sBt);
   if (le < EPW && e0 + le < mesh.nelem)
   {
 #pragma unroll
@@ -520,7 +583,7 @@ __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, double mu,
         (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
   }
   __syncthreads();
-  produce_traces<P, Q>(mesh, mu, sPT, sQT, sV, sX, sR, e0, Tout);
+  produce_traces<P, Q>(mesh, ph, sPT, sQT, sV, sX, sNb, sBt, sR, e0, Tout, status);
 }
 
 // One full right-hand-side evaluation f_g = B3 f of EPW elements fused with
@@ -535,7 +598,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
                           int* status)
 {
   using FD = FaceDims<P>;
-  constexpr int N = FD::N, NB = FD::NB, NF = FD::NF, EPW = FD::EPW;
+  constexpr int N = FD::N, NB = FD::NB, NF = FD::NF, EPW = FD::EPW, FS = FD::FS;
   constexpr int NG = (Q + 1) * (Q + 1) * (Q + 1);
 
   __shared__ double sV[EPW][5][NB];
@@ -543,6 +606,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
   __shared__ double sR[FD::RG];  // volume flux of one direction, then one face pass
   __shared__ PTab sPT;
   __shared__ QTab sQT;
+  __shared__ int sNb[EPW][6], sBt[EPW][6];
   stage_tables<P, Q>(&sPT, &sQT);
   const PTab& pt = sPT;
   const QTab& qt = sQT;
@@ -553,6 +617,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
   const int e0  = blockIdx.x * EPW;
   const int e   = e0 + le;
   const bool vact = (le < EPW) && (e < mesh.nelem);
+  face_table<EPW>(mesh, e0, sNb, sBt);
 
   // ---- load state and element geometry ----
   if (vact)
@@ -658,8 +723,8 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
   // ---- faces, in passes of FPP whole faces ----
   // Both traces of a face point come from the trace records Tin written with
   // the input state (own face f, and the neighbour's face f^1 at the same
-  // face point).  Per face point in sR (stride 20): G (0-4), normal
-  // contravariant dcQ (5-8), tangential (10-13, 15-18); the T phase then
+  // face point).  Per face point in sR (stride FS = 17): G (0-4), normal
+  // contravariant dcQ (5-8), tangential (9-12, 13-16); the T phase then
   // replaces G by T = G - Dg^T H_t1 - Dg^T H_t2.
   bool ok = true;
   for (int pass = 0; pass < FD::NPASS; ++pass)
@@ -678,9 +743,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
     double hn[4], h1[4], h2[4];
     if (fact)
     {
-      int bt = -1;
-      const int en = neighbour(mesh, ee % mesh.nx, (ee / mesh.nx) % mesh.ny,
-                               ee / (mesh.nx * mesh.ny), f, &bt);
+      const int en = sNb[l][f];
       const double* To = sa.Tin + ((size_t)ee * 6 + f) * 8 * NF + fpt;
       double UL[5], vgL[3];
 #pragma unroll
@@ -701,7 +764,8 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
       const double fJ = face_normal(J, nd, hi, nv);
       const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
 
-      // neighbour trace; its traction was taken with its own outward normal -n
+      // neighbour trace (its traction was taken with its own outward normal
+      // -n), or the boundary state stored with the traces
       double Ub[5], vgR[3] = {0.0, 0.0, 0.0};
       double lamR = 0.0;
       if (en >= 0)
@@ -716,7 +780,12 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
         lamR = lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
       }
       else
-        ok = boundary_state(bt, UL, nv, ph, Ub) && ok;
+      {
+        const double* B = sa.Tin + (size_t)mesh.nelem * 6 * 8 * NF + (size_t)(-1 - en) * 5 * NF + fpt;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          Ub[r] = B[r * NF];
+      }
       face_point_physics(en >= 0, UL, Ub, vgL, vgR, nv, lamL, lamR, ph, fpo);
 
       // contravariant dual-consistency flux h_c = Ji[c] . dcQ (rows 1-4)
@@ -745,19 +814,19 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
       for (int r = 0; r < 5; ++r)
       {
         fpo.G[r] *= sc;
-        sR[tid * 20 + r] = fpo.G[r];
+        sR[tid * FS + r] = fpo.G[r];
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r)
       {
-        sR[tid * 20 + 5 + r]  = hn[r];
-        sR[tid * 20 + 10 + r] = h1[r];
-        sR[tid * 20 + 15 + r] = h2[r];
+        sR[tid * FS + 5 + r]  = hn[r];
+        sR[tid * FS + 9 + r]  = h1[r];
+        sR[tid * FS + 13 + r] = h2[r];
       }
     }
     __syncthreads();
     // T = G - Dg^T(t1) H_t1 - Dg^T(t2) H_t2 (mass row has no viscous part);
-    // only slots 10-18 of other points are read, so T overwrites 0-4
+    // only slots 9-16 of other points are read, so T overwrites 0-4
     if (fact)
     {
 #pragma unroll
@@ -767,10 +836,10 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
 #pragma unroll
         for (int m = 0; m < N; ++m)
         {
-          t -= pt.Dg[m][a] * sR[(fbase + m + N * b) * 20 + 9 + r];
-          t -= pt.Dg[m][b] * sR[(fbase + a + N * m) * 20 + 14 + r];
+          t -= pt.Dg[m][a] * sR[(fbase + m + N * b) * FS + 8 + r];
+          t -= pt.Dg[m][b] * sR[(fbase + a + N * m) * FS + 12 + r];
         }
-        sR[tid * 20 + r] = t;
+        sR[tid * FS + r] = t;
       }
     }
     __syncthreads();
@@ -788,10 +857,10 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
         int ft2 = (fnd == 2) ? 1 : 2;
         int m  = idx[fnd];
         int fp = (g % FD::FPP) * NF + idx[ft2] * N + idx[ft1];
-        acc[0] += pt.L[fhi][m] * sR[fp * 20];
+        acc[0] += pt.L[fhi][m] * sR[fp * FS];
 #pragma unroll
         for (int r = 1; r < 5; ++r)
-          acc[r] += pt.L[fhi][m] * sR[fp * 20 + r] - pt.Ld[fhi][m] * sR[fp * 20 + 4 + r];
+          acc[r] += pt.L[fhi][m] * sR[fp * FS + r] - pt.Ld[fhi][m] * sR[fp * FS + 4 + r];
       }
     }
     __syncthreads();
@@ -836,7 +905,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
   if (sa.Tout)
   {
     __syncthreads();
-    produce_traces<P, Q>(mesh, ph.mu, pt, qt, sV, sX, sR, e0, sa.Tout);
+    produce_traces<P, Q>(mesh, ph, pt, qt, sV, sX, sNb, sBt, sR, e0, sa.Tout, status);
   }
 }
 
@@ -1168,12 +1237,13 @@ hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, co
 }
 
 template <int P, int Q>
-hipError_t launch_trace(const MeshArgs& m, double mu, const double* V, double* Tout,
-                        hipStream_t st)
+hipError_t launch_trace(const MeshArgs& m, const Phys& ph, const double* V, double* Tout,
+                        int* status, hipStream_t st)
 {
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
-  hipLaunchKernelGGL((dgx_trace_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, mu, V, Tout);
+  hipLaunchKernelGGL((dgx_trace_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, V, Tout,
+                     status);
   return hipGetLastError();
 }
 
@@ -1206,6 +1276,12 @@ hipError_t launch_resid_out(const MeshArgs& m, const double* fg, double* f, doub
   return hipGetLastError();
 }
 
+size_t trace_doubles(int p, int nx, int ny, int nz)
+{
+  size_t nf = (size_t)(p + 1) * (p + 1);
+  return (size_t)nx * ny * nz * 6 * 8 * nf + boundary_faces(nx, ny, nz) * 5 * nf;
+}
+
 int resid_out_blocks(int p, int nelem)
 {
   int nb = (p + 1) * (p + 1) * (p + 1);
@@ -1236,10 +1312,10 @@ hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const S
 #undef CALL_RHS
 }
 
-hipError_t trace_dispatch(int p, int q, const MeshArgs& m, double mu, const double* V,
-                          double* Tout, hipStream_t st)
+hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
+                          double* Tout, int* status, hipStream_t st)
 {
-#define CALL_TR(P, Q) launch_trace<P, Q>(m, mu, V, Tout, st)
+#define CALL_TR(P, Q) launch_trace<P, Q>(m, ph, V, Tout, status, st)
   DGX_DISPATCH_PQ(p, q, CALL_TR)
 #undef CALL_TR
 }

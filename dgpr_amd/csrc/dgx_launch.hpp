@@ -77,9 +77,13 @@ struct StageArgs
 // SIMD), chosen by default_stage_waves or by timing (dgx_autotune)
 hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
                         const double* lam, int* status, int waves, hipStream_t st);
-// face trace records Tout of the Gauss-nodal state V
-hipError_t trace_dispatch(int p, int q, const MeshArgs& m, double mu, const double* V,
-                          double* Tout, hipStream_t st);
+// face trace records Tout of the Gauss-nodal state V (and the boundary
+// states of its domain-boundary faces, see trace_doubles)
+hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
+                          double* Tout, int* status, hipStream_t st);
+// doubles of one trace record set: nelem*6*8*(p+1)^2 trace records followed
+// by 5*(p+1)^2 boundary-state values for each of the mesh's boundary faces
+size_t trace_doubles(int p, int nx, int ny, int nz);
 // lam[e][f][(p+1)^2]: BR2 lifting coefficient of each element side of each face
 hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipStream_t st);
 hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg, double* f,

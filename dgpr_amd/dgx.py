@@ -43,6 +43,7 @@ SCHEMES = {"RK4": RK4, "tvdRK3": TVDRK3}
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "HIP error",
           4: "NaN residual norm", 5: "I/O error", 6: "no HIP device"}
+OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_NAN, ERR_IO, ERR_NODEVICE = range(7)
 
 
 class DgxError(RuntimeError):

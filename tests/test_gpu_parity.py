@@ -58,6 +58,8 @@ CASES = {
     "tgv4_p1": lambda: cases.tgv_case(n=4, p=1, seed=11, nsteps=5),
     "tgv4_p2": lambda: cases.tgv_case(n=4, p=2, seed=12, nsteps=5),
     "tgv3_p3": lambda: cases.tgv_case(n=3, p=3, seed=13, nsteps=5),
+    # long horizon: 100 RK4 steps at the CFL of the 64^3 workload (dt c/h)
+    "tgv4_p3_long": lambda: cases.tgv_case(n=4, p=3, seed=25, dt=4e-3, nsteps=100),
     "tgv_curved_p3": lambda: cases.tgv_case(nxyz=(3, 4, 2), p=3, seed=14, amp=0.15, nsteps=3),
     "tgv_curved_p2": lambda: cases.tgv_case(nxyz=(4, 3, 3), p=2, seed=24, amp=0.2, nsteps=3),
     "channel_p1": lambda: cases.channel_case(4, 4, 3, p=1, seed=15, nsteps=5),
@@ -172,3 +174,31 @@ def test_occupancy_targets_agree(orc, built):
         assert np.all(us > 0)
         U_before = s.get_state()
         assert rel(U_before, case.U0) < 1e-14  # tuning launches leave the state alone
+
+
+def test_inflow_without_mach_root_is_reported(orc, built):
+    """A state leaving the domain through a subsonic-inflow face so fast that
+    the boundary Mach quadratic has no non-negative root (the reference aborts
+    on the host, bound_states.cpp:80-83) is reported as DGX_ERR_INVALID by
+    both the residual and the stepper; the context stays usable."""
+    case = bump_case(2, 1, DEFAULT_BOUNDS, 31)
+    m = case.mesh
+    bad = np.zeros((m.nelem, 5, m.nbf))
+    rho, u, p = 1.0, -200.0, 0.01
+    bad[:, 0] = rho
+    bad[:, 1] = rho * u
+    bad[:, 4] = p / 0.4 + 0.5 * rho * u * u
+    with dgx.Solver(m, case.params) as s:
+        s.set_state(bad.reshape(-1))
+        with pytest.raises(dgx.DgxError) as ei:
+            s.residual()
+        assert ei.value.status == dgx.ERR_INVALID
+        s.set_state(bad.reshape(-1))
+        with pytest.raises(dgx.DgxError) as ei:
+            s.step(case.dt, case.scheme, 1)
+        assert ei.value.status == dgx.ERR_INVALID
+        s.set_state(case.U0)
+        _, f = s.residual()
+        geo = orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
+        _, f_o = geo.residual(orc_params(orc, case.params), case.U0)
+        assert rel(f, f_o) < RES_TOL
