@@ -44,6 +44,7 @@ struct dgx_ctx
   double* TUb;
   bool tu_valid;    // TU holds the traces of the current U
   int stage_waves;  // occupancy target of the stage kernel (1..3)
+  double* mean;     // running mean (Gauss-nodal), allocated on first use
   int* status;      // kernel-side error flags
   int npartial;
   size_t bytes;
@@ -275,8 +276,8 @@ dgx_status dgx_destroy(dgx_ctx* c)
     return DGX_OK;
   if (c->stream)
     (void)hipStreamSynchronize(c->stream);
-  double* bufs[12] = {c->U,       c->Ua,  c->Ub,  c->acc, c->fs,  c->nodes,
-                      c->partial, c->sum, c->lam, c->TU,  c->TUa, c->TUb};
+  double* bufs[13] = {c->U,   c->Ua,  c->Ub,  c->acc, c->fs,  c->nodes, c->partial,
+                      c->sum, c->lam, c->TU,  c->TUa, c->TUb, c->mean};
   for (double* b : bufs)
     if (b)
       (void)hipFree(b);
@@ -614,4 +615,63 @@ dgx_status dgx_autotune(dgx_ctx* c, int reps, int* chosen, double us_out[3])
   if (st != DGX_OK)
     return st;
   return check_status(c);
+}
+
+dgx_status dgx_sample_output_device(dgx_ctx* c, int var, int N, double* out_dev)
+{
+  if (!c || !out_dev || var < DGX_OUT_MACH || var > DGX_OUT_W || N < 1 || N > 64)
+    return DGX_ERR_INVALID;
+  HIPCHK(c, sample_dispatch(c->p, c->U, c->nelem, N, var, c->params.gamma, out_dev, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
+dgx_status dgx_sample_output(dgx_ctx* c, int var, int N, double* out_host)
+{
+  if (!c || !out_host || var < DGX_OUT_MACH || var > DGX_OUT_W || N < 1 || N > 64)
+    return DGX_ERR_INVALID;
+  size_t n = (size_t)c->nelem * (N + 1) * (N + 1) * (N + 1);
+  double* d = nullptr;
+  HIPCHK(c, hipMalloc(&d, n * sizeof(double)));
+  hipError_t e = sample_dispatch(c->p, c->U, c->nelem, N, var, c->params.gamma, d, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out_host, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return fail(c, DGX_ERR_HIP, "sample_output: %s", hipGetErrorString(e));
+  return DGX_OK;
+}
+
+dgx_status dgx_update_mean(dgx_ctx* c, uint64_t time_step, uint64_t mean_start)
+{
+  if (!c || time_step < mean_start)
+    return DGX_ERR_INVALID;
+  size_t sb = c->nstate * sizeof(double);
+  if (!c->mean)
+  {
+    HIPCHK(c, hipMalloc(&c->mean, sb));
+    c->bytes += sb;
+    HIPCHK(c, hipMemsetAsync(c->mean, 0, sb, c->stream));
+  }
+  else if (time_step == mean_start)
+    HIPCHK(c, hipMemsetAsync(c->mean, 0, sb, c->stream));
+  double old_range = (double)(time_step - mean_start);
+  double new_range = (double)(time_step + 1 - mean_start);
+  HIPCHK(c, mean_dispatch(c->U, c->mean, c->nstate, old_range, new_range, c->stream));
+  return DGX_OK;
+}
+
+dgx_status dgx_get_mean(dgx_ctx* c, double* mean_host)
+{
+  if (!c || !mean_host)
+    return DGX_ERR_INVALID;
+  if (!c->mean)
+    return fail(c, DGX_ERR_INVALID, "no running mean yet (call dgx_update_mean first)");
+  HIPCHK(c, basis_dispatch(c->p, c->mean, c->Ua, c->nelem, 1, c->stream));
+  HIPCHK(c, hipMemcpyAsync(mean_host, c->Ua, c->nstate * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
 }

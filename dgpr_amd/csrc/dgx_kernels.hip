@@ -1214,6 +1214,85 @@ hipError_t copy_dispatch(const void* src, void* dst, size_t bytes, hipStream_t s
   return hipGetLastError();
 }
 
+// ---- derived outputs -------------------------------------------------------
+
+// eval_output (render/vrtxgen.cpp:25-59) on the (Ns+1)^3 equispaced sub-grid
+// of every element, reference points (i/Ns, j/Ns, k/Ns) with sub-node index
+// (Ns+1)^2 k + (Ns+1) j + i (vrtxgen.cpp:294-330).  The Gauss-nodal state is
+// interpolated with the Gauss-node Lagrange basis, which spans the same
+// polynomials as the reference's equispaced basis.  var: mach (0), rho (1),
+// u (2), v (3), w (4); the Mach number keeps the reference's pressure
+// (gamma - 1)(rho E - |u|^2 / 2).  One thread per sample point.
+template <int P>
+__global__ __launch_bounds__(256) void dgx_sample_kernel(const double* __restrict__ V,
+                                                         int nelem, int Ns, int var,
+                                                         double gamma,
+                                                         double* __restrict__ out)
+{
+  constexpr int N = P + 1, NB = N * N * N;
+  const PTab& pt = c_ptab[P - 1];
+  const int np1 = Ns + 1, nsub = np1 * np1 * np1;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nelem * nsub)
+    return;
+  const int node = (int)(t % nsub);
+  const size_t e = t / nsub;
+  const int ijk[3] = {node % np1, (node / np1) % np1, node / (np1 * np1)};
+  double l[3][N];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+  {
+    const double x = (double)ijk[c] / (double)Ns;
+#pragma unroll
+    for (int m = 0; m < N; ++m)
+    {
+      double v = 1.0;
+#pragma unroll
+      for (int o = 0; o < N; ++o)
+        if (o != m)
+          v *= (x - pt.g[o]) / (pt.g[m] - pt.g[o]);
+      l[c][m] = v;
+    }
+  }
+  double st[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const double* Ve = V + e * 5 * NB;
+#pragma unroll
+  for (int kk = 0; kk < N; ++kk)
+#pragma unroll
+    for (int jj = 0; jj < N; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < N; ++ii)
+      {
+        const double w = l[2][kk] * l[1][jj] * l[0][ii];
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          st[r] += w * Ve[r * NB + nid<N>(ii, jj, kk)];
+      }
+  double v = 0.0;
+  if (var == 0)
+  {
+    const double u = st[1] / st[0], vv = st[2] / st[0], ww = st[3] / st[0];
+    const double sp = sqrt(u * u + vv * vv + ww * ww);
+    const double pr = (gamma - 1.0) * (st[4] - 0.5 * sp * sp);
+    v = sp / sqrt(gamma * pr / st[0]);
+  }
+  else if (var == 1)
+    v = st[0];
+  else
+    v = st[var - 1] / st[0];
+  out[t] = v;
+}
+
+// update_mean (output.cpp:39-49): mean = (mean * old_range + U) / new_range.
+__global__ __launch_bounds__(256) void dgx_mean_kernel(const double* __restrict__ U,
+                                                       double* __restrict__ mean, size_t n,
+                                                       double old_range, double new_range)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    mean[i] = (mean[i] * old_range + U[i]) / new_range;
+}
+
 // ---- launch helpers (host) -----------------------------------------------
 
 template <int P, int Q>
@@ -1345,6 +1424,39 @@ hipError_t basis_dispatch(int p, const double* in, double* out, int nelem, int m
     case 3: return launch_basis<3>(in, out, nelem, mode, st);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t sample_dispatch(int p, const double* V, int nelem, int Ns, int var, double gamma,
+                           double* out, hipStream_t st)
+{
+  size_t total = (size_t)nelem * (Ns + 1) * (Ns + 1) * (Ns + 1);
+  int nblk = (int)((total + 255) / 256);
+  switch (p)
+  {
+    case 1:
+      hipLaunchKernelGGL(dgx_sample_kernel<1>, dim3(nblk), dim3(256), 0, st, V, nelem, Ns, var,
+                         gamma, out);
+      break;
+    case 2:
+      hipLaunchKernelGGL(dgx_sample_kernel<2>, dim3(nblk), dim3(256), 0, st, V, nelem, Ns, var,
+                         gamma, out);
+      break;
+    case 3:
+      hipLaunchKernelGGL(dgx_sample_kernel<3>, dim3(nblk), dim3(256), 0, st, V, nelem, Ns, var,
+                         gamma, out);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t mean_dispatch(const double* U, double* mean, size_t n, double old_range,
+                         double new_range, hipStream_t st)
+{
+  int nblk = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(dgx_mean_kernel, dim3(nblk), dim3(256), 0, st, U, mean, n, old_range,
+                     new_range);
+  return hipGetLastError();
 }
 
 hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st)

@@ -91,6 +91,12 @@ hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg,
 hipError_t basis_dispatch(int p, const double* in, double* out, int nelem, int mode,
                           hipStream_t st);
 hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st);
+// derived outputs of a Gauss-nodal state on the (Ns+1)^3 sub-grid of each element
+hipError_t sample_dispatch(int p, const double* V, int nelem, int Ns, int var, double gamma,
+                           double* out, hipStream_t st);
+// running mean update of n doubles
+hipError_t mean_dispatch(const double* U, double* mean, size_t n, double old_range,
+                         double new_range, hipStream_t st);
 hipError_t upload_tables(const PTab* pt, const QTab* qt);
 hipError_t copy_dispatch(const void* src, void* dst, size_t bytes, hipStream_t st);
 int resid_out_blocks(int p, int nelem);

@@ -44,6 +44,8 @@ SCHEMES = {"RK4": RK4, "tvdRK3": TVDRK3}
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "HIP error",
           4: "NaN residual norm", 5: "I/O error", 6: "no HIP device"}
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_NAN, ERR_IO, ERR_NODEVICE = range(7)
+# output_type (output.cpp:23-30)
+OUTPUTS = {"mach": 0, "rho": 1, "u": 2, "v": 3, "w": 4}
 
 
 class DgxError(RuntimeError):
@@ -107,6 +109,10 @@ def lib():
         L.dgx_read_dg_header.argtypes = [C.c_char_p, C.POINTER(U64), C.POINTER(_CMesh)]
         L.dgx_read_dg.argtypes = [C.c_char_p, PD, PD]
         L.dgx_set_tuning.argtypes = [P_ctx, C.c_int]
+        L.dgx_sample_output.argtypes = [P_ctx, C.c_int, C.c_int, PD]
+        L.dgx_sample_output_device.argtypes = [P_ctx, C.c_int, C.c_int, C.c_void_p]
+        L.dgx_update_mean.argtypes = [P_ctx, U64, U64]
+        L.dgx_get_mean.argtypes = [P_ctx, PD]
         L.dgx_get_tuning.argtypes = [P_ctx, C.POINTER(C.c_int)]
         L.dgx_autotune.argtypes = [P_ctx, C.c_int, C.POINTER(C.c_int), PD]
         for nm in ("dgx_create", "dgx_destroy", "dgx_set_params", "dgx_set_state",
@@ -115,7 +121,8 @@ def lib():
                    "dgx_sync", "dgx_get_stream", "dgx_get_sizes", "dgx_make_reference_state",
                    "dgx_bump_nodes", "dgx_box_nodes", "dgx_tgv_state", "dgx_channel_state",
                    "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg", "dgx_set_tuning",
-                   "dgx_get_tuning", "dgx_autotune"):
+                   "dgx_get_tuning", "dgx_autotune", "dgx_sample_output",
+                   "dgx_sample_output_device", "dgx_update_mean", "dgx_get_mean"):
             getattr(L, nm).restype = st
         _lib = L
     return _lib
@@ -320,6 +327,23 @@ class Solver:
     @stage_waves.setter
     def stage_waves(self, w):
         _chk(lib().dgx_set_tuning(self.ctx, int(w)), self.ctx)
+
+    def sample_output(self, var, N):
+        """eval_output (render/vrtxgen.cpp:25-59) of the current state on the
+        (N+1)^3 equispaced sub-grid of every element: [nelem][(N+1)^3]."""
+        v = OUTPUTS[var] if isinstance(var, str) else int(var)
+        out = np.zeros(self.mesh.nelem * (N + 1) ** 3)
+        _chk(lib().dgx_sample_output(self.ctx, v, int(N), _p(out)), self.ctx)
+        return out.reshape(self.mesh.nelem, (N + 1) ** 3)
+
+    def update_mean(self, time_step, mean_start):
+        """update_mean (output.cpp:39-49) on the current state."""
+        _chk(lib().dgx_update_mean(self.ctx, int(time_step), int(mean_start)), self.ctx)
+
+    def get_mean(self):
+        M = np.zeros(self.nstate)
+        _chk(lib().dgx_get_mean(self.ctx, _p(M)), self.ctx)
+        return M
 
     def autotune(self, reps=5):
         """Times each compiled occupancy target on this mesh, keeps the

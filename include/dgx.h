@@ -171,6 +171,38 @@ dgx_status dgx_get_sizes(const dgx_ctx* ctx, uint64_t out[4]);
  * failed dgx_create when ctx is NULL). */
 const char* dgx_last_error(const dgx_ctx* ctx);
 
+/* ---- derived outputs ----------------------------------------------------- */
+
+/* output_type (output.cpp:23-30) */
+enum
+{
+  DGX_OUT_MACH = 0,
+  DGX_OUT_RHO  = 1,
+  DGX_OUT_U    = 2,
+  DGX_OUT_V    = 3,
+  DGX_OUT_W    = 4
+};
+
+/* Replaces eval_output (render/vrtxgen.cpp:25-59) over the sub-grid of
+ * compute_metadata_cpu (vrtxgen.cpp:294-330): the output var of the
+ * context's state at the (N+1)^3 equispaced reference points (i/N, j/N, k/N)
+ * of every element, out[e][(N+1)^2 k + (N+1) j + i].  The Mach number keeps
+ * the reference's pressure (gamma - 1)(rho E - |u|^2 / 2).  1 <= N <= 64.
+ * out_dev is a device buffer of nelem*(N+1)^3 doubles; the host form copies
+ * the result back. */
+dgx_status dgx_sample_output_device(dgx_ctx* ctx, int var, int N, double* out_dev);
+dgx_status dgx_sample_output(dgx_ctx* ctx, int var, int N, double* out_host);
+
+/* Replaces update_mean(U, mean, time_step, mean_start) of the driver loop
+ * (output.cpp:39-49, dg.cpp:123-131) on the context's current state: the
+ * mean is kept on the device and restarts (zeroed, then updated) when
+ * time_step == mean_start.  time_step < mean_start is DGX_ERR_INVALID. */
+dgx_status dgx_update_mean(dgx_ctx* ctx, uint64_t time_step, uint64_t mean_start);
+
+/* The running mean in simstate layout (DGX_ERR_INVALID before the first
+ * dgx_update_mean). */
+dgx_status dgx_get_mean(dgx_ctx* ctx, double* mean_host);
+
 /* ---- launch tuning (no counterpart in the reference, whose kernels have
  * fixed launch shapes, solve_cuda.cu:1235-1282) ---------------------------- */
 

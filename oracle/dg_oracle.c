@@ -1564,3 +1564,66 @@ void orc_step_generic(int scheme, double dt, u64 N, double* U, orc_rhs_fn rhs,
     free(U1);
   }
 }
+
+/* ---- derived outputs ------------------------------------------------------ */
+
+/* eval_output (render/vrtxgen.cpp:25-59) on the (N+1)^3 equispaced sub-grid
+ * of every element (reference points (i/N, j/N, k/N), sub-node index
+ * (N+1)^2 k + (N+1) j + i, vrtxgen.cpp:294-330): the state is evaluated
+ * from its equispaced-Lagrange coefficients and var selects mach (0), rho
+ * (1), u (2), v (3), w (4).  The Mach number keeps the reference's pressure
+ * (gamma - 1)(rho E - |u|^2 / 2), without the density factor. */
+void orc_eval_output(int p, u64 nelem, const double* U, int N, int var, double gamma,
+                     double* out)
+{
+  int n = p + 1, nbf = n * n * n, np1 = N + 1;
+  double runit = 1.0 / (double)N;
+  double* phi = (double*)zalloc(sizeof(double) * (size_t)np1 * np1 * np1 * nbf);
+  for (int k = 0; k < np1; ++k)
+    for (int j = 0; j < np1; ++j)
+      for (int i = 0; i < np1; ++i)
+      {
+        double r[3] = {(double)i * runit, (double)j * runit, (double)k * runit};
+        int node = np1 * np1 * k + np1 * j + i;
+        for (int b = 0; b < nbf; ++b)
+          phi[(size_t)nbf * node + b] = lagrange3d(b, p, r);
+      }
+  for (u64 e = 0; e < nelem; ++e)
+    for (int node = 0; node < np1 * np1 * np1; ++node)
+    {
+      double s[5] = {0, 0, 0, 0, 0};
+      for (int b = 0; b < nbf; ++b)
+      {
+        double ev = phi[(size_t)nbf * node + b];
+        for (int r = 0; r < 5; ++r)
+          s[r] += U[((size_t)e * 5 + r) * nbf + b] * ev;
+      }
+      double v = 0.0;
+      switch (var)
+      {
+        case 0: {
+          double u = s[1] / s[0], vv = s[2] / s[0], w = s[3] / s[0];
+          double sp = sqrt(u * u + vv * vv + w * w);
+          double pr = (gamma - 1.0) * (s[4] - 0.5 * sp * sp);
+          double c = sqrt(gamma * pr / s[0]);
+          v = sp / c;
+        }
+        break;
+        case 1: v = s[0]; break;
+        case 2: v = s[1] / s[0]; break;
+        case 3: v = s[2] / s[0]; break;
+        case 4: v = s[3] / s[0]; break;
+      }
+      out[(size_t)e * np1 * np1 * np1 + node] = v;
+    }
+  free(phi);
+}
+
+/* update_mean (output.cpp:39-49): running mean of the state since mean_start. */
+void orc_update_mean(const double* state, double* mean, u64 n, u64 time_step, u64 mean_start)
+{
+  double old_range = (double)(time_step - mean_start);
+  double new_range = (double)(time_step + 1 - mean_start);
+  for (u64 i = 0; i < n; ++i)
+    mean[i] = (mean[i] * old_range + state[i]) / new_range;
+}

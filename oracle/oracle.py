@@ -75,6 +75,8 @@ def lib():
         L.orc_residual_norm.argtypes = [PD, U64]
         L.orc_residual_norm.restype = D
         L.orc_set_plant.argtypes = [C.c_int]
+        L.orc_eval_output.argtypes = [C.c_int, U64, PD, C.c_int, C.c_int, D, PD]
+        L.orc_update_mean.argtypes = [PD, PD, U64, U64, U64]
         L.RHS = C.CFUNCTYPE(None, PD, PD, C.c_void_p)
         L.orc_step_generic.argtypes = [C.c_int, D, U64, PD, L.RHS, C.c_void_p]
     return _lib
@@ -353,3 +355,23 @@ def params_from(prm):
         o.Ufr[i] = prm.Ufr[i]
         o.source[i] = prm.source[i]
     return o
+
+
+OUTPUTS = {"mach": 0, "rho": 1, "u": 2, "v": 3, "w": 4}
+
+
+def eval_output(p, U, nelem, N, var, gamma):
+    """eval_output (render/vrtxgen.cpp:25-59) on the (N+1)^3 equispaced
+    sub-grid of every element; returns [nelem][(N+1)^3]."""
+    U = _f64(U)
+    out = np.zeros(nelem * (N + 1) ** 3)
+    lib().orc_eval_output(p, nelem, _p(U), N, OUTPUTS.get(var, var), gamma, _p(out))
+    return out.reshape(nelem, (N + 1) ** 3)
+
+
+def update_mean(state, mean, time_step, mean_start):
+    """update_mean (output.cpp:39-49), in place on a copy; returns it."""
+    state = _f64(state)
+    mean = _f64(mean).copy()
+    lib().orc_update_mean(_p(state), _p(mean), state.size, time_step, mean_start)
+    return mean
