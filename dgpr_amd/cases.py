@@ -98,3 +98,29 @@ def config2():
     the reference residual as well as on the device, while dt c / h halved
     stays bounded over 220 steps."""
     return tgv_case(n=64, p=3, seed=2, dt=2.5e-4, nsteps=100, scheme="RK4")
+
+
+def ensemble_cases(ncase=64, seed=3, nsteps=50, cfl=0.2):
+    """Independent periodic Taylor-Green cases drawn from ``seed``: n uniform in
+    {8, 12, 16, 24, 32} (n^3 elements), p uniform in {1, 2, 3}, Mach uniform in
+    [0.05, 0.3], Re log-uniform in [400, 5000], each with seeded velocity
+    noise; RK4 with dt = cfl h / ((V0 + c)(2p + 1)) per case.  cfl = 0.2 is
+    the 64^3 p=3 workload's stable dt (2.5e-4) expressed in the same form."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(ncase):
+        n = int(rng.choice([8, 12, 16, 24, 32]))
+        p = int(rng.integers(1, 4))
+        M = float(rng.uniform(0.05, 0.3))
+        Re = float(math.exp(rng.uniform(math.log(400.0), math.log(5000.0))))
+        c = 1.0 / M
+        dt = cfl * (TWO_PI / n) / ((1.0 + c) * (2 * p + 1))
+        out.append(tgv_case(n=n, p=p, seed=1000 * seed + i, M=M, Re=Re, dt=dt,
+                            nsteps=nsteps, scheme="RK4"))
+    return out
+
+
+def config3():
+    """64 independent periodic cubes (sizes, orders, Mach and Re from seed 3),
+    FP64 RK4, 50 steps each."""
+    return ensemble_cases()

@@ -16,40 +16,11 @@
 #include <vector>
 
 #include "../../include/dgx.h"
+#include "dgx_ctx.hpp"
 #include "dgx_launch.hpp"
 
 using namespace dgx;
 
-struct dgx_ctx
-{
-  int device;
-  int p, q;
-  int nx, ny, nz, nelem, nbf;
-  int bounds[6];
-  size_t nstate;  // nelem*5*nbf
-  dgx_params params;
-  Phys phys;
-  hipStream_t stream;
-  double* nodes;    // [e][(q+1)^3][3]
-  double* U;        // state, Gauss-nodal basis
-  double* Ua;       // stage buffer / staging
-  double* Ub;       // stage buffer
-  double* acc;      // RK4 accumulator
-  double* fs;       // f_g scratch
-  double* partial;  // per-block sums of R^2
-  double* sum;      // reduced sum
-  double* lam;      // BR2 lifting coefficients [e][6][(p+1)^2]
-  double* TU;       // face trace records of U, Ua, Ub [e][6][8][(p+1)^2]
-  double* TUa;
-  double* TUb;
-  bool tu_valid;    // TU holds the traces of the current U
-  int stage_waves;  // occupancy target of the stage kernel (1..3)
-  double* mean;     // running mean (Gauss-nodal), allocated on first use
-  int* status;      // kernel-side error flags
-  int npartial;
-  size_t bytes;
-  char err[512];
-};
 
 static thread_local char g_create_err[512] = "";
 static std::mutex g_tab_mutex;
@@ -75,7 +46,7 @@ static dgx_status fail(dgx_ctx* c, dgx_status st, const char* fmt, ...)
 
 // Occupancy target per (p, q): 3 waves/SIMD where the stage kernel fits the
 // 168-register budget without spilling (p=3, q=1), 2 elsewhere.
-static int default_stage_waves(int p, int q)
+int dgx::default_stage_waves(int p, int q)
 {
   return (p == 3 && q == 1) ? 3 : 2;
 }
@@ -100,7 +71,7 @@ static void derive_phys(const dgx_params& p, Phys& ph)
   }
 }
 
-static MeshArgs mesh_args(const dgx_ctx* c)
+MeshArgs dgx::mesh_args(const dgx_ctx* c)
 {
   MeshArgs m;
   m.nodes = c->nodes;
@@ -392,8 +363,10 @@ static hipError_t rhs(dgx_ctx* c, const double* Vin, const double* Tin, const do
                       c->stream);
 }
 
+}  // extern "C"
+
 // Trace records of U, recomputed when U was set from outside.
-static hipError_t ensure_traces(dgx_ctx* c)
+hipError_t dgx::ensure_traces(dgx_ctx* c)
 {
   if (c->tu_valid)
     return hipSuccess;
@@ -403,6 +376,8 @@ static hipError_t ensure_traces(dgx_ctx* c)
     c->tu_valid = true;
   return e;
 }
+
+extern "C" {
 
 static dgx_status check_status(dgx_ctx* c)
 {
@@ -445,64 +420,74 @@ dgx_status dgx_residual(dgx_ctx* c, double* R_host, double* f_host)
   return check_status(c);
 }
 
-// Launches one step; if want_norm, stores sum(R^2) of stage 1 in c->sum.
-// Every stage writes the trace records of its output state, so the next
-// stage (and the next step) reads them instead of recomputing.
+}  // extern "C"
+
+// Stage arguments of one step (time_distributed.cpp:46-102, classic RK4 with
+// an accumulator, Shu-Osher TVD-RK3).  Every stage writes the trace records
+// of its output state, so the next stage (and the next step) reads them
+// instead of recomputing.
+int dgx::stage_plan(const dgx_ctx* c, int scheme, double dt, StageArgs sa[4])
+{
+  auto set = [](StageArgs& a, const double* Vin, const double* Tin, const double* Vbase,
+                double* acc, double* Vout, double* Tout, double ca, double cb, double cf,
+                int accmode) {
+    a.Vin = Vin;
+    a.Tin = Tin;
+    a.Vbase = Vbase;
+    a.acc = acc;
+    a.Vout = Vout;
+    a.Tout = Tout;
+    a.fout = nullptr;
+    a.ca = ca;
+    a.cb = cb;
+    a.cf = cf;
+    a.accmode = accmode;
+  };
+  if (scheme == DGX_RK4)
+  {
+    set(sa[0], c->U, c->TU, c->U, c->acc, c->Ua, c->TUa, 1.0, 0.0, 0.5 * dt, 1);  // U1, acc = f1
+    set(sa[1], c->Ua, c->TUa, c->U, c->acc, c->Ub, c->TUb, 1.0, 0.0, 0.5 * dt, 2);  // U2, acc += 2 f2
+    set(sa[2], c->Ub, c->TUb, c->U, c->acc, c->Ua, c->TUa, 1.0, 0.0, dt, 2);  // U3, acc += 2 f3
+    set(sa[3], c->Ua, c->TUa, c->U, c->acc, c->U, c->TU, 1.0, 0.0, dt, 3);  // U += dt/6 (acc + f4)
+    return 4;
+  }
+  set(sa[0], c->U, c->TU, c->U, nullptr, c->Ua, c->TUa, 1.0, 0.0, dt, 0);  // U1 = U + dt f
+  set(sa[1], c->Ua, c->TUa, c->U, nullptr, c->Ub, c->TUb, 0.75, 0.25, 0.25 * dt, 0);
+  set(sa[2], c->Ub, c->TUb, c->U, nullptr, c->U, c->TU, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt,
+      0);
+  return 3;
+}
+
+extern "C" {
+
+// Launches one step; if want_norm, stores sum(R^2) of stage 1 in c->sum
+// (RK4: from the accumulator, which holds f1; TVD-RK3: from f written by the
+// first stage).
 static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm)
 {
   hipError_t e;
   if ((e = ensure_traces(c)) != hipSuccess)
     return e;
   c->tu_valid = false;  // until the last stage has rewritten them
-  if (scheme == DGX_RK4)
+  StageArgs sa[4];
+  int ns = stage_plan(c, scheme, dt, sa);
+  for (int k = 0; k < ns; ++k)
   {
-    // U1 = U + dt/2 f1 ; acc = f1
-    if ((e = rhs(c, c->U, c->TU, c->U, c->acc, c->Ua, c->TUa, nullptr, 1.0, 0.0, 0.5 * dt, 1)) !=
-        hipSuccess)
+    if (k == 0 && want_norm && scheme != DGX_RK4)
+      sa[0].fout = c->fs;
+    if ((e = rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa[k], c->lam, c->status,
+                          c->stage_waves, c->stream)) != hipSuccess)
       return e;
-    if (want_norm)
+    if (k == 0 && want_norm)
     {
-      if ((e = resid_out_dispatch(c->p, c->q, mesh_args(c), c->acc, nullptr, nullptr,
-                                  c->partial, c->stream)) != hipSuccess)
+      const double* f1 = (scheme == DGX_RK4) ? c->acc : c->fs;
+      if ((e = resid_out_dispatch(c->p, c->q, mesh_args(c), f1, nullptr, nullptr, c->partial,
+                                  c->stream)) != hipSuccess)
         return e;
       if ((e = sum_dispatch(c->partial, c->npartial, c->sum, c->stream)) != hipSuccess)
         return e;
     }
-    // U2 = U + dt/2 f2 ; acc += 2 f2
-    if ((e = rhs(c, c->Ua, c->TUa, c->U, c->acc, c->Ub, c->TUb, nullptr, 1.0, 0.0, 0.5 * dt,
-                 2)) != hipSuccess)
-      return e;
-    // U3 = U + dt f3 ; acc += 2 f3
-    if ((e = rhs(c, c->Ub, c->TUb, c->U, c->acc, c->Ua, c->TUa, nullptr, 1.0, 0.0, dt, 2)) !=
-        hipSuccess)
-      return e;
-    // U = U + dt/6 (acc + f4)
-    if ((e = rhs(c, c->Ua, c->TUa, c->U, c->acc, c->U, c->TU, nullptr, 1.0, 0.0, dt, 3)) !=
-        hipSuccess)
-      return e;
-    c->tu_valid = true;
-    return hipSuccess;
   }
-  // TVD-RK3: U1 = U + dt f
-  if ((e = rhs(c, c->U, c->TU, c->U, nullptr, c->Ua, c->TUa, want_norm ? c->fs : nullptr, 1.0,
-               0.0, dt, 0)) != hipSuccess)
-    return e;
-  if (want_norm)
-  {
-    if ((e = resid_out_dispatch(c->p, c->q, mesh_args(c), c->fs, nullptr, nullptr, c->partial,
-                                c->stream)) != hipSuccess)
-      return e;
-    if ((e = sum_dispatch(c->partial, c->npartial, c->sum, c->stream)) != hipSuccess)
-      return e;
-  }
-  // U2 = 3/4 U + 1/4 U1 + 1/4 dt f
-  if ((e = rhs(c, c->Ua, c->TUa, c->U, nullptr, c->Ub, c->TUb, nullptr, 0.75, 0.25, 0.25 * dt,
-               0)) != hipSuccess)
-    return e;
-  // U = 1/3 U + 2/3 U2 + 2/3 dt f
-  if ((e = rhs(c, c->Ub, c->TUb, c->U, nullptr, c->U, c->TU, nullptr, 1.0 / 3.0, 2.0 / 3.0,
-               (2.0 / 3.0) * dt, 0)) != hipSuccess)
-    return e;
   c->tu_valid = true;
   return hipSuccess;
 }

@@ -1,0 +1,389 @@
+// dgx_ensemble.cpp -- batched ensembles of independent cases.
+//
+// The reference runs one case per process (dg.cpp / cudg.cu).  An ensemble
+// keeps one ordinary context per member (mesh, state, stage buffers, trace
+// records, lifting coefficients) and advances all members together: for every
+// RK stage, the members that share (p, q) run as ONE launch of the batched
+// stage kernel (each grid block finds its member in a sorted item table), so
+// small members no longer leave most of the GPU idle.  One RK step of the
+// whole ensemble (stages x groups launches) can be captured in a hipGraph and
+// replayed.  Each group's occupancy target is chosen by timing
+// (dgx_ensemble_autotune).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/dgx.h"
+#include "dgx_ctx.hpp"
+#include "dgx_launch.hpp"
+
+using namespace dgx;
+
+namespace
+{
+struct EnsGroup
+{
+  int p, q;
+  std::vector<int> members;
+  int nblocks;
+  int waves;
+  BatchItem* items[4];  // device item tables, one per stage
+};
+}  // namespace
+
+struct dgx_ensemble
+{
+  int device;
+  hipStream_t stream;
+  std::vector<dgx_ctx*> ctx;
+  std::vector<EnsGroup> groups;
+  int* status;
+  // stage items are built for one (scheme, dt[]) at a time
+  bool planned;
+  int scheme;
+  int nstages;
+  std::vector<double> dt;
+  bool graph_ready;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  char err[512];
+};
+
+static thread_local char g_ens_err[512] = "";
+
+static dgx_status efail(dgx_ensemble* en, dgx_status st, const char* fmt, ...)
+{
+  char* buf = en ? en->err : g_ens_err;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, 512, fmt, ap);
+  va_end(ap);
+  return st;
+}
+
+#define EHIP(en, call)                                                                     \
+  do                                                                                       \
+  {                                                                                        \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return efail((en), DGX_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
+  } while (0)
+
+static void drop_graph(dgx_ensemble* en)
+{
+  if (en->graph_ready)
+  {
+    (void)hipGraphExecDestroy(en->exec);
+    (void)hipGraphDestroy(en->graph);
+    en->graph_ready = false;
+  }
+}
+
+// Builds the per-stage item tables of every group for (scheme, dt[]).
+static dgx_status plan(dgx_ensemble* en, int scheme, const double* dt)
+{
+  const int n = (int)en->ctx.size();
+  if (en->planned && en->scheme == scheme &&
+      std::memcmp(en->dt.data(), dt, sizeof(double) * n) == 0)
+    return DGX_OK;
+  drop_graph(en);
+  int ns = 0;
+  for (EnsGroup& g : en->groups)
+  {
+    std::vector<BatchItem> host[4];
+    int block0 = 0;
+    for (int m : g.members)
+    {
+      dgx_ctx* c = en->ctx[m];
+      StageArgs sa[4];
+      ns = stage_plan(c, scheme, dt[m], sa);
+      for (int s = 0; s < ns; ++s)
+      {
+        BatchItem it;
+        it.mesh = mesh_args(c);
+        it.ph = c->phys;
+        it.sa = sa[s];
+        it.lam = c->lam;
+        it.block0 = block0;
+        host[s].push_back(it);
+      }
+      block0 += stage_blocks(c->p, c->nelem);
+    }
+    g.nblocks = block0;
+    EHIP(en, hipStreamSynchronize(en->stream));  // no launch may still read the tables
+    for (int s = 0; s < ns; ++s)
+      EHIP(en, hipMemcpy(g.items[s], host[s].data(), sizeof(BatchItem) * host[s].size(),
+                         hipMemcpyHostToDevice));
+  }
+  en->planned = true;
+  en->scheme = scheme;
+  en->nstages = ns;
+  en->dt.assign(dt, dt + n);
+  return DGX_OK;
+}
+
+static hipError_t enqueue_stage(dgx_ensemble* en, int s, int only_group = -1)
+{
+  for (int gi = 0; gi < (int)en->groups.size(); ++gi)
+  {
+    if (only_group >= 0 && gi != only_group)
+      continue;
+    EnsGroup& g = en->groups[gi];
+    hipError_t e = rhs_batch_dispatch(g.p, g.q, g.items[s], (int)g.members.size(), g.nblocks,
+                                      en->status, g.waves, en->stream);
+    if (e != hipSuccess)
+      return e;
+  }
+  return hipSuccess;
+}
+
+static hipError_t enqueue_step(dgx_ensemble* en)
+{
+  for (int s = 0; s < en->nstages; ++s)
+  {
+    hipError_t e = enqueue_stage(en, s);
+    if (e != hipSuccess)
+      return e;
+  }
+  return hipSuccess;
+}
+
+// Trace records of members whose state was set from outside; every member
+// stream is idle afterwards.
+static dgx_status prepare_members(dgx_ensemble* en)
+{
+  for (dgx_ctx* c : en->ctx)
+  {
+    EHIP(en, ensure_traces(c));
+    EHIP(en, hipStreamSynchronize(c->stream));
+  }
+  return DGX_OK;
+}
+
+extern "C" {
+
+dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_params* params,
+                               dgx_ensemble** out)
+{
+  if (ncase < 1 || !meshes || !params || !out)
+    return efail(nullptr, DGX_ERR_INVALID, "invalid ensemble arguments");
+  *out = nullptr;
+  dgx_ensemble* en = new dgx_ensemble();
+  en->planned = false;
+  en->graph_ready = false;
+  en->status = nullptr;
+  en->stream = nullptr;
+  en->err[0] = 0;
+  for (int i = 0; i < ncase; ++i)
+  {
+    dgx_ctx* c = nullptr;
+    dgx_status st = dgx_create(&meshes[i], &params[i], &c);
+    if (st != DGX_OK)
+    {
+      efail(nullptr, st, "member %d: %s", i, dgx_last_error(nullptr));
+      dgx_ensemble_destroy(en);
+      return st;
+    }
+    en->ctx.push_back(c);
+  }
+  en->device = en->ctx[0]->device;
+  for (int i = 0; i < ncase; ++i)
+  {
+    dgx_ctx* c = en->ctx[i];
+    EnsGroup* g = nullptr;
+    for (EnsGroup& x : en->groups)
+      if (x.p == c->p && x.q == c->q)
+        g = &x;
+    if (!g)
+    {
+      EnsGroup ng;
+      ng.p = c->p;
+      ng.q = c->q;
+      ng.nblocks = 0;
+      ng.waves = default_stage_waves(c->p, c->q);
+      for (int s = 0; s < 4; ++s)
+        ng.items[s] = nullptr;
+      en->groups.push_back(ng);
+      g = &en->groups.back();
+    }
+    g->members.push_back(i);
+  }
+  auto bail = [&](const char* what, hipError_t e) {
+    snprintf(g_ens_err, sizeof(g_ens_err), "%s failed: %s", what, hipGetErrorString(e));
+    dgx_ensemble_destroy(en);
+    return DGX_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&en->stream, hipStreamNonBlocking)) != hipSuccess)
+    return bail("hipStreamCreate", e);
+  for (EnsGroup& g : en->groups)
+    for (int s = 0; s < 4; ++s)
+      if ((e = hipMalloc(&g.items[s], sizeof(BatchItem) * g.members.size())) != hipSuccess)
+        return bail("hipMalloc(items)", e);
+  if ((e = hipMalloc(&en->status, sizeof(int))) != hipSuccess)
+    return bail("hipMalloc(status)", e);
+  if ((e = hipMemsetAsync(en->status, 0, sizeof(int), en->stream)) != hipSuccess)
+    return bail("hipMemset(status)", e);
+  if ((e = hipStreamSynchronize(en->stream)) != hipSuccess)
+    return bail("hipStreamSynchronize", e);
+  *out = en;
+  return DGX_OK;
+}
+
+dgx_status dgx_ensemble_destroy(dgx_ensemble* en)
+{
+  if (!en)
+    return DGX_OK;
+  if (en->stream)
+    (void)hipStreamSynchronize(en->stream);
+  drop_graph(en);
+  for (EnsGroup& g : en->groups)
+    for (int s = 0; s < 4; ++s)
+      if (g.items[s])
+        (void)hipFree(g.items[s]);
+  if (en->status)
+    (void)hipFree(en->status);
+  for (dgx_ctx* c : en->ctx)
+    dgx_destroy(c);
+  if (en->stream)
+    (void)hipStreamDestroy(en->stream);
+  delete en;
+  return DGX_OK;
+}
+
+dgx_status dgx_ensemble_member(dgx_ensemble* en, int i, dgx_ctx** ctx)
+{
+  if (!en || !ctx || i < 0 || i >= (int)en->ctx.size())
+    return DGX_ERR_INVALID;
+  *ctx = en->ctx[i];
+  return DGX_OK;
+}
+
+dgx_status dgx_ensemble_step(dgx_ensemble* en, int scheme, const double* dt, uint64_t nsteps,
+                             int use_graph)
+{
+  if (!en || !dt || (scheme != DGX_RK4 && scheme != DGX_TVDRK3))
+    return DGX_ERR_INVALID;
+  dgx_status st = plan(en, scheme, dt);
+  if (st != DGX_OK)
+    return st;
+  if ((st = prepare_members(en)) != DGX_OK)
+    return st;
+  for (dgx_ctx* c : en->ctx)
+    c->tu_valid = false;
+  if (use_graph && !en->graph_ready && nsteps > 0)
+  {
+    EHIP(en, hipStreamBeginCapture(en->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_step(en);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(en->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess)
+    {
+      if (g)
+        (void)hipGraphDestroy(g);
+      return efail(en, DGX_ERR_HIP, "graph capture failed: %s",
+                   hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    hipGraphExec_t x;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess)
+    {
+      (void)hipGraphDestroy(g);
+      return efail(en, DGX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    en->graph = g;
+    en->exec = x;
+    en->graph_ready = true;
+  }
+  for (uint64_t k = 0; k < nsteps; ++k)
+  {
+    if (use_graph)
+      EHIP(en, hipGraphLaunch(en->exec, en->stream));
+    else
+      EHIP(en, enqueue_step(en));
+  }
+  for (dgx_ctx* c : en->ctx)
+    c->tu_valid = true;
+  return DGX_OK;
+}
+
+dgx_status dgx_ensemble_sync(dgx_ensemble* en)
+{
+  if (!en)
+    return DGX_ERR_INVALID;
+  EHIP(en, hipStreamSynchronize(en->stream));
+  int h = 0;
+  EHIP(en, hipMemcpy(&h, en->status, sizeof(int), hipMemcpyDeviceToHost));
+  if (h)
+  {
+    EHIP(en, hipMemset(en->status, 0, sizeof(int)));
+    return efail(en, DGX_ERR_INVALID, "negative inflow boundary Mach number in a member");
+  }
+  return DGX_OK;
+}
+
+dgx_status dgx_ensemble_get_stream(dgx_ensemble* en, void** stream)
+{
+  if (!en || !stream)
+    return DGX_ERR_INVALID;
+  *stream = (void*)en->stream;
+  return DGX_OK;
+}
+
+dgx_status dgx_ensemble_autotune(dgx_ensemble* en, int scheme, const double* dt, int reps,
+                                 int* waves_out)
+{
+  if (!en || !dt || reps < 1 || (scheme != DGX_RK4 && scheme != DGX_TVDRK3))
+    return DGX_ERR_INVALID;
+  dgx_status st = plan(en, scheme, dt);
+  if (st != DGX_OK)
+    return st;
+  if ((st = prepare_members(en)) != DGX_OK)
+    return st;
+  drop_graph(en);
+  hipEvent_t e0, e1;
+  EHIP(en, hipEventCreate(&e0));
+  EHIP(en, hipEventCreate(&e1));
+  // stage 1 reads U and its records and writes only stage buffers
+  for (int gi = 0; gi < (int)en->groups.size(); ++gi)
+  {
+    EnsGroup& g = en->groups[gi];
+    int best = g.waves;
+    double best_ms = 0.0;
+    for (int w = 1; w <= 3; ++w)
+    {
+      g.waves = w;
+      EHIP(en, enqueue_stage(en, 0, gi));
+      EHIP(en, hipEventRecord(e0, en->stream));
+      for (int r = 0; r < reps; ++r)
+        EHIP(en, enqueue_stage(en, 0, gi));
+      EHIP(en, hipEventRecord(e1, en->stream));
+      EHIP(en, hipEventSynchronize(e1));
+      float ms = 0.0f;
+      EHIP(en, hipEventElapsedTime(&ms, e0, e1));
+      if (w == 1 || ms < best_ms)
+      {
+        best_ms = ms;
+        best = w;
+      }
+    }
+    g.waves = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (waves_out)
+    for (const EnsGroup& g : en->groups)
+      for (int m : g.members)
+        waves_out[m] = g.waves;
+  return dgx_ensemble_sync(en);
+}
+
+const char* dgx_ensemble_last_error(const dgx_ensemble* en)
+{
+  return en ? en->err : g_ens_err;
+}
+
+}  // extern "C"

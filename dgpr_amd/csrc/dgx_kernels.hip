@@ -590,12 +590,12 @@ __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
 // the RK stage combination of StageArgs, reading the face trace records Tin
 // of Vin and (optionally) writing those of Vout to Tout.  lam[e][f][pt]
 // holds the BR2 lifting coefficient of element e's side of its face f.
-// W is the occupancy target (waves per SIMD) handed to the register
-// allocator; the launch picks one of the compiled targets per (P, Q).
-template <int P, int Q, int W>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
-void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
-                          int* status)
+// Body of the stage kernel for block `block` of one mesh: shared by the
+// single-mesh kernel and the batched ensemble kernel.
+template <int P, int Q>
+__device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
+                                           const StageArgs& sa, const double* __restrict__ lam,
+                                           int* status, int block)
 {
   using FD = FaceDims<P>;
   constexpr int N = FD::N, NB = FD::NB, NF = FD::NF, EPW = FD::EPW, FS = FD::FS;
@@ -614,7 +614,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
   const int tid = threadIdx.x;
   const int le  = tid / NB;
   const int nd0 = tid % NB;
-  const int e0  = blockIdx.x * EPW;
+  const int e0  = block * EPW;
   const int e   = e0 + le;
   const bool vact = (le < EPW) && (e < mesh.nelem);
   face_table<EPW>(mesh, e0, sNb, sBt);
@@ -907,6 +907,37 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
     __syncthreads();
     produce_traces<P, Q>(mesh, ph, pt, qt, sV, sX, sNb, sBt, sR, e0, sa.Tout, status);
   }
+}
+
+// W is the occupancy target (waves per SIMD) handed to the register
+// allocator; the launch picks one of the compiled targets per (P, Q).
+template <int P, int Q, int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
+void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
+                          int* status)
+{
+  stage_body<P, Q>(mesh, ph, sa, lam, status, blockIdx.x);
+}
+
+// Batched form: the blocks of several independent meshes of the same (P, Q)
+// in one launch.  items[] is sorted by first block; each block finds its
+// member by a uniform binary search.
+template <int P, int Q, int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
+void dgx_rhs_batch_kernel(const BatchItem* __restrict__ items, int nitems, int* status)
+{
+  const int b = blockIdx.x;
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const BatchItem& it = items[lo];
+  stage_body<P, Q>(it.mesh, it.ph, it.sa, it.lam, status, b - it.block0);
 }
 
 // BR2 lifting coefficient of each element's side of each of its faces:
@@ -1316,6 +1347,24 @@ hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, co
 }
 
 template <int P, int Q>
+hipError_t launch_rhs_batch(const BatchItem* items, int nitems, int nblk, int* status, int waves,
+                            hipStream_t st)
+{
+  if (waves == 1)
+    hipLaunchKernelGGL((dgx_rhs_batch_kernel<P, Q, 1>), dim3(nblk), dim3(64), 0, st, items,
+                       nitems, status);
+  else if (waves == 2)
+    hipLaunchKernelGGL((dgx_rhs_batch_kernel<P, Q, 2>), dim3(nblk), dim3(64), 0, st, items,
+                       nitems, status);
+  else if (waves == 3)
+    hipLaunchKernelGGL((dgx_rhs_batch_kernel<P, Q, 3>), dim3(nblk), dim3(64), 0, st, items,
+                       nitems, status);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int P, int Q>
 hipError_t launch_trace(const MeshArgs& m, const Phys& ph, const double* V, double* Tout,
                         int* status, hipStream_t st)
 {
@@ -1389,6 +1438,20 @@ hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const S
 #define CALL_RHS(P, Q) launch_rhs<P, Q>(m, ph, sa, lam, status, waves, st)
   DGX_DISPATCH_PQ(p, q, CALL_RHS)
 #undef CALL_RHS
+}
+
+hipError_t rhs_batch_dispatch(int p, int q, const BatchItem* items_dev, int nitems, int nblocks,
+                              int* status, int waves, hipStream_t st)
+{
+#define CALL_RB(P, Q) launch_rhs_batch<P, Q>(items_dev, nitems, nblocks, status, waves, st)
+  DGX_DISPATCH_PQ(p, q, CALL_RB)
+#undef CALL_RB
+}
+
+int stage_blocks(int p, int nelem)
+{
+  int epw = 64 / ((p + 1) * (p + 1) * (p + 1));
+  return (nelem + epw - 1) / epw;
 }
 
 hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,

@@ -73,10 +73,27 @@ struct StageArgs
   int accmode;
 };
 
+// One member of a batched stage launch: its mesh, physics, stage arguments
+// and lifting coefficients, and the first grid block of its elements.
+struct BatchItem
+{
+  MeshArgs mesh;
+  Phys ph;
+  StageArgs sa;
+  const double* lam;
+  int block0;
+};
+
 // waves: occupancy target of the compiled stage kernel (1, 2 or 3 waves per
 // SIMD), chosen by default_stage_waves or by timing (dgx_autotune)
 hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
                         const double* lam, int* status, int waves, hipStream_t st);
+// one stage of several meshes of the same (p, q): items_dev (device memory,
+// sorted by block0) covers nblocks blocks in total
+hipError_t rhs_batch_dispatch(int p, int q, const BatchItem* items_dev, int nitems, int nblocks,
+                              int* status, int waves, hipStream_t st);
+// grid blocks of one mesh in a stage launch
+int stage_blocks(int p, int nelem);
 // face trace records Tout of the Gauss-nodal state V (and the boundary
 // states of its domain-boundary faces, see trace_doubles)
 hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,

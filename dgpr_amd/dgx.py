@@ -109,6 +109,17 @@ def lib():
         L.dgx_read_dg_header.argtypes = [C.c_char_p, C.POINTER(U64), C.POINTER(_CMesh)]
         L.dgx_read_dg.argtypes = [C.c_char_p, PD, PD]
         L.dgx_set_tuning.argtypes = [P_ctx, C.c_int]
+        L.dgx_ensemble_create.argtypes = [C.c_int, C.POINTER(_CMesh), C.POINTER(Params),
+                                          C.POINTER(C.c_void_p)]
+        L.dgx_ensemble_destroy.argtypes = [C.c_void_p]
+        L.dgx_ensemble_member.argtypes = [C.c_void_p, C.c_int, C.POINTER(P_ctx)]
+        L.dgx_ensemble_step.argtypes = [C.c_void_p, C.c_int, PD, U64, C.c_int]
+        L.dgx_ensemble_sync.argtypes = [C.c_void_p]
+        L.dgx_ensemble_get_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        L.dgx_ensemble_autotune.argtypes = [C.c_void_p, C.c_int, PD, C.c_int,
+                                            C.POINTER(C.c_int)]
+        L.dgx_ensemble_last_error.argtypes = [C.c_void_p]
+        L.dgx_ensemble_last_error.restype = C.c_char_p
         L.dgx_sample_output.argtypes = [P_ctx, C.c_int, C.c_int, PD]
         L.dgx_sample_output_device.argtypes = [P_ctx, C.c_int, C.c_int, C.c_void_p]
         L.dgx_update_mean.argtypes = [P_ctx, U64, U64]
@@ -122,7 +133,10 @@ def lib():
                    "dgx_bump_nodes", "dgx_box_nodes", "dgx_tgv_state", "dgx_channel_state",
                    "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg", "dgx_set_tuning",
                    "dgx_get_tuning", "dgx_autotune", "dgx_sample_output",
-                   "dgx_sample_output_device", "dgx_update_mean", "dgx_get_mean"):
+                   "dgx_sample_output_device", "dgx_update_mean", "dgx_get_mean",
+                   "dgx_ensemble_create", "dgx_ensemble_destroy", "dgx_ensemble_member",
+                   "dgx_ensemble_step", "dgx_ensemble_sync", "dgx_ensemble_get_stream",
+                   "dgx_ensemble_autotune"):
             getattr(L, nm).restype = st
         _lib = L
     return _lib
@@ -352,3 +366,84 @@ class Solver:
         us = np.zeros(3)
         _chk(lib().dgx_autotune(self.ctx, int(reps), C.byref(w), _p(us)), self.ctx)
         return w.value, us
+
+
+class _Member(Solver):
+    """A Solver view of an ensemble member's context (owned by the ensemble)."""
+
+    def __init__(self, ctx, mesh, params):  # noqa: D401 -- no dgx_create here
+        self.ctx = ctx
+        self.mesh = mesh
+        self.params = params.copy()
+        self.nelem, self.nbf = mesh.nelem, mesh.nbf
+        self.nstate = mesh.nelem * 5 * mesh.nbf
+
+    def close(self):
+        self.ctx = None
+
+
+class Ensemble:
+    """Independent cases advanced together (dgx_ensemble_*): members with the
+    same (p, q) share one batched launch per RK stage; ``step(graph=True)``
+    replays one captured step of the whole ensemble."""
+
+    def __init__(self, meshes, params):
+        n = len(meshes)
+        cm = (_CMesh * n)(*[m.cstruct() for m in meshes])
+        cp = (Params * n)(*[p.copy() for p in params])
+        h = C.c_void_p()
+        st = lib().dgx_ensemble_create(n, cm, cp, C.byref(h))
+        if st != 0:
+            raise DgxError(st, lib().dgx_ensemble_last_error(None).decode())
+        self.h = h
+        self.meshes = list(meshes)
+        self.members = []
+        for i, (m, p) in enumerate(zip(meshes, params)):
+            c = C.c_void_p()
+            self._chk(lib().dgx_ensemble_member(self.h, i, C.byref(c)))
+            self.members.append(_Member(c, m, p))
+
+    def _chk(self, st):
+        if st != 0:
+            raise DgxError(st, lib().dgx_ensemble_last_error(self.h).decode())
+
+    def step(self, scheme, dt, nsteps, graph=True, sync=True):
+        sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        dts = np.ascontiguousarray(dt, dtype=np.float64)
+        self._chk(lib().dgx_ensemble_step(self.h, sc, _p(dts), int(nsteps), int(bool(graph))))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        self._chk(lib().dgx_ensemble_sync(self.h))
+
+    def stream(self):
+        s = C.c_void_p()
+        self._chk(lib().dgx_ensemble_get_stream(self.h, C.byref(s)))
+        return s.value
+
+    def autotune(self, scheme, dt, reps=3):
+        sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        dts = np.ascontiguousarray(dt, dtype=np.float64)
+        w = (C.c_int * len(self.members))()
+        self._chk(lib().dgx_ensemble_autotune(self.h, sc, _p(dts), int(reps), w))
+        return list(w)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.close()
+            lib().dgx_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

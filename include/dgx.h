@@ -203,6 +203,35 @@ dgx_status dgx_update_mean(dgx_ctx* ctx, uint64_t time_step, uint64_t mean_start
  * dgx_update_mean). */
 dgx_status dgx_get_mean(dgx_ctx* ctx, double* mean_host);
 
+/* ---- batched ensembles (no counterpart in the reference, which runs one
+ * case per process, dg.cpp / cudg.cu) ----------------------------------------
+ * An ensemble owns one context per member (meshes of any size, p and q; own
+ * parameters).  dgx_ensemble_step advances every member by nsteps steps of
+ * its own dt: per RK stage, the members sharing (p, q) run as one launch of
+ * the batched stage kernel; with use_graph != 0 one step of the whole
+ * ensemble is captured once in a hipGraph and replayed.  Results equal those
+ * of dgx_step on each member.  The call only enqueues work on the ensemble's
+ * stream: call dgx_ensemble_sync before using a member context again. */
+typedef struct dgx_ensemble dgx_ensemble;
+dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_params* params,
+                               dgx_ensemble** out);
+dgx_status dgx_ensemble_destroy(dgx_ensemble* ens);
+/* The context of member i (set/get its state, residual, single-case steps). */
+dgx_status dgx_ensemble_member(dgx_ensemble* ens, int i, dgx_ctx** ctx);
+dgx_status dgx_ensemble_step(dgx_ensemble* ens, int scheme, const double* dt, uint64_t nsteps,
+                             int use_graph);
+/* Waits for the ensemble's stream; DGX_ERR_INVALID if a member met a
+ * subsonic inflow without a Mach root. */
+dgx_status dgx_ensemble_sync(dgx_ensemble* ens);
+dgx_status dgx_ensemble_get_stream(dgx_ensemble* ens, void** stream);
+/* Times the batched stage kernel of every (p, q) group at each occupancy
+ * target (reps launches of the first stage, which writes stage buffers
+ * only), keeps the fastest per group and writes each member's choice to
+ * waves_out[ncase] (may be NULL). */
+dgx_status dgx_ensemble_autotune(dgx_ensemble* ens, int scheme, const double* dt, int reps,
+                                 int* waves_out);
+const char* dgx_ensemble_last_error(const dgx_ensemble* ens);
+
 /* ---- launch tuning (no counterpart in the reference, whose kernels have
  * fixed launch shapes, solve_cuda.cu:1235-1282) ---------------------------- */
 
