@@ -15,8 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdgx.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["dgx_kernels.hip", "dgx_api.cpp", "dgx_ensemble.cpp", "dgx_tables.cpp",
-           "dgx_host.cpp"]
+SOURCES = ["dgx_kernels.hip", "dgx_api.cpp", "dgx_ensemble.cpp", "dgx_adjoint.cpp",
+           "dgx_tables.cpp", "dgx_host.cpp"]
 ARCH = os.environ.get("DGX_ARCH", "gfx950")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

@@ -1,0 +1,326 @@
+// dgx_adjoint.cpp -- discrete adjoint of the residual and of classic RK4
+// (no counterpart in the reference, which has no adjoint).
+//
+// Everything is the exact transpose of what the device computes (the same
+// Gauss-nodal residual f_g, the stage algebra of stage_plan), so the
+// gradients are those of the discrete forward map.  Periodic meshes only:
+// the reversed face kernels assume every face has a neighbour.
+//
+// RK4 step (time_distributed.cpp:46-76): S1 = V + dt/2 k1, S2 = V + dt/2 k2,
+// S3 = V + dt k3, V' = V + dt/6 (k1 + 2 k2 + 2 k3 + k4), k_i = f(S_(i-1)).
+// Reverse, with l the adjoint of V' and a_i = f_V(S_i)^T (adjoint of k_(i+1)):
+//   l4 = dt/6 l,            a3 = f_V(S3)^T l4
+//   l3 = dt/3 l + dt a3,    a2 = f_V(S2)^T l3
+//   l2 = dt/3 l + dt/2 a2,  a1 = f_V(S1)^T l2
+//   l1 = dt/6 l + dt/2 a1,  a0 = f_V(V)^T l1
+//   adjoint of V = l + a0 + a1 + a2 + a3,
+// and dJ/dmu gains sum_i l_i . df/dmu(S_(i-1)), df/dmu = (f(S; mu) - f(S; 0))/mu
+// because every viscous and BR2 term is linear in mu.  Checkpoints every
+// `interval` steps (default ceil(sqrt(N))); each segment is recomputed once
+// and its step-start states kept, so the reverse sweep costs one extra
+// forward run plus, per step, three stage recomputations and four
+// residual adjoints.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/dgx.h"
+#include "dgx_ctx.hpp"
+#include "dgx_launch.hpp"
+
+using namespace dgx;
+
+namespace
+{
+struct DevBuf
+{
+  double* p = nullptr;
+  ~DevBuf()
+  {
+    if (p)
+      (void)hipFree(p);
+  }
+};
+
+dgx_status afail(dgx_ctx* c, dgx_status st, const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(c->err, sizeof(c->err), fmt, ap);
+  va_end(ap);
+  return st;
+}
+
+#define AHIP(c, call)                                                                      \
+  do                                                                                       \
+  {                                                                                        \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return afail((c), DGX_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));       \
+  } while (0)
+
+bool all_periodic(const dgx_ctx* c)
+{
+  for (int i = 0; i < 6; ++i)
+    if (c->bounds[i] != DGX_BT_PERIODIC)
+      return false;
+  return true;
+}
+
+// Workspace of the residual adjoint
+struct AdjScratch
+{
+  DevBuf pvol, aown, anbr;
+};
+
+hipError_t alloc(DevBuf& b, size_t ndouble)
+{
+  return hipMalloc(&b.p, ndouble * sizeof(double));
+}
+
+hipError_t alloc_scratch(const dgx_ctx* c, AdjScratch& s)
+{
+  size_t tr = trace_doubles(c->p, c->nx, c->ny, c->nz);
+  hipError_t e = alloc(s.pvol, c->nstate);
+  if (e == hipSuccess)
+    e = alloc(s.aown, tr);
+  if (e == hipSuccess)
+    e = alloc(s.anbr, tr);
+  return e;
+}
+
+// one stage kernel launch with explicit physics: Vout = Vbase + cf f(Vin)
+// (Vout may be null), fout = f(Vin) (may be null)
+hipError_t stage(dgx_ctx* c, const Phys& ph, const double* Vin, const double* Tin,
+                 const double* Vbase, double* Vout, double* Tout, double* fout, double cf)
+{
+  StageArgs sa;
+  sa.Vin = Vin;
+  sa.Tin = Tin;
+  sa.Vbase = Vbase;
+  sa.acc = nullptr;
+  sa.Vout = Vout;
+  sa.Tout = Tout;
+  sa.fout = fout;
+  sa.ca = 1.0;
+  sa.cb = 0.0;
+  sa.cf = cf;
+  sa.accmode = 0;
+  return rhs_dispatch(c->p, c->q, mesh_args(c), ph, sa, c->lam, c->status, c->stage_waves,
+                      c->stream);
+}
+
+// one RK4 step of the context's state (the dgx_step path)
+hipError_t step_ctx(dgx_ctx* c, double dt)
+{
+  hipError_t e = ensure_traces(c);
+  if (e != hipSuccess)
+    return e;
+  c->tu_valid = false;
+  StageArgs sa[4];
+  int ns = stage_plan(c, DGX_RK4, dt, sa);
+  for (int k = 0; k < ns && e == hipSuccess; ++k)
+    e = rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa[k], c->lam, c->status,
+                     c->stage_waves, c->stream);
+  if (e == hipSuccess)
+    c->tu_valid = true;
+  return e;
+}
+
+hipError_t copy_state(dgx_ctx* c, double* dst, const double* src)
+{
+  return hipMemcpyAsync(dst, src, c->nstate * sizeof(double), hipMemcpyDeviceToDevice,
+                        c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+dgx_status dgx_kinetic_energy(dgx_ctx* c, double* J)
+{
+  if (!c || !J)
+    return DGX_ERR_INVALID;
+  DevBuf g;
+  AHIP(c, alloc(g, c->nstate));
+  AHIP(c, kinetic_energy_dispatch(c->p, c->q, mesh_args(c), c->U, g.p, c->partial, c->stream));
+  AHIP(c, sum_dispatch(c->partial, c->npartial, c->sum, c->stream));
+  AHIP(c, hipMemcpyAsync(J, c->sum, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  AHIP(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
+dgx_status dgx_residual_vjp(dgx_ctx* c, const double* w_host, double* g_host)
+{
+  if (!c || !w_host || !g_host)
+    return DGX_ERR_INVALID;
+  if (!all_periodic(c))
+    return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs a periodic mesh");
+  AdjScratch s;
+  DevBuf a, b;
+  AHIP(c, alloc_scratch(c, s));
+  AHIP(c, alloc(a, c->nstate));
+  AHIP(c, alloc(b, c->nstate));
+  AHIP(c, ensure_traces(c));
+  // f = B3^-1 f_g(B3 U): (df/dU)^T w = B3^T (df_g/dV)^T B3^-T w
+  AHIP(c, hipMemcpyAsync(a.p, w_host, c->nstate * sizeof(double), hipMemcpyHostToDevice,
+                         c->stream));
+  AHIP(c, basis_dispatch(c->p, a.p, b.p, c->nelem, 3, c->stream));
+  AHIP(c, adjoint_dispatch(c->p, c->q, mesh_args(c), c->phys, c->U, c->TU, c->lam, b.p,
+                           s.pvol.p, s.aown.p, s.anbr.p, a.p, c->stream));
+  AHIP(c, basis_dispatch(c->p, a.p, b.p, c->nelem, 2, c->stream));
+  AHIP(c, hipMemcpyAsync(g_host, b.p, c->nstate * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  AHIP(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
+dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t interval, double* J,
+                           double* dJdU0_host, double* dJdmu)
+{
+  if (!c || nsteps == 0 || !(dt > 0.0))
+    return DGX_ERR_INVALID;
+  if (!all_periodic(c))
+    return afail(c, DGX_ERR_UNSUPPORTED, "the discrete adjoint needs a periodic mesh");
+  if (dJdmu && !(c->phys.mu > 0.0))
+    return afail(c, DGX_ERR_INVALID, "dJ/dmu needs mu > 0");
+  const uint64_t K = interval ? interval : (uint64_t)std::ceil(std::sqrt((double)nsteps));
+  const uint64_t nchk = (nsteps + K - 1) / K;
+  const size_t n = c->nstate;
+  const size_t ntr = trace_doubles(c->p, c->nx, c->ny, c->nz);
+
+  std::vector<DevBuf> chk(nchk), seg(K);
+  for (auto& b : chk)
+    AHIP(c, alloc(b, n));
+  for (auto& b : seg)
+    AHIP(c, alloc(b, n));
+  DevBuf S[4], T[4], kf[4], lam, lk, acur, asum, tz, f0, dpart, dsum, final_state;
+  for (int i = 0; i < 4; ++i)
+  {
+    AHIP(c, alloc(S[i], n));
+    AHIP(c, alloc(T[i], ntr));
+    AHIP(c, alloc(kf[i], n));
+  }
+  AHIP(c, alloc(lam, n));
+  AHIP(c, alloc(lk, n));
+  AHIP(c, alloc(acur, n));
+  AHIP(c, alloc(asum, n));
+  AHIP(c, alloc(final_state, n));
+  if (dJdmu)
+  {
+    AHIP(c, alloc(tz, ntr));
+    AHIP(c, alloc(f0, n));
+  }
+  const int ndot = 256;
+  AHIP(c, alloc(dpart, ndot));
+  AHIP(c, alloc(dsum, 1));
+  AdjScratch scr;
+  AHIP(c, alloc_scratch(c, scr));
+  Phys ph0 = c->phys;
+  ph0.mu = 0.0;
+
+  // forward sweep with checkpoints
+  for (uint64_t s = 0; s < nsteps; ++s)
+  {
+    if (s % K == 0)
+      AHIP(c, copy_state(c, chk[s / K].p, c->U));
+    AHIP(c, step_ctx(c, dt));
+  }
+  AHIP(c, copy_state(c, final_state.p, c->U));
+  // objective and its gradient at the final state
+  AHIP(c, kinetic_energy_dispatch(c->p, c->q, mesh_args(c), c->U, lam.p, c->partial, c->stream));
+  AHIP(c, sum_dispatch(c->partial, c->npartial, c->sum, c->stream));
+  double Jv = 0.0;
+  AHIP(c, hipMemcpyAsync(&Jv, c->sum, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+
+  double dmu_total = 0.0;
+  std::vector<double> dmu_parts;
+  const double cstage[3] = {0.5 * dt, 0.5 * dt, dt};
+  for (uint64_t sg = nchk; sg-- > 0;)
+  {
+    const uint64_t s0 = sg * K, s1 = (s0 + K < nsteps) ? s0 + K : nsteps;
+    // recompute the segment's step-start states
+    AHIP(c, copy_state(c, c->U, chk[sg].p));
+    c->tu_valid = false;
+    for (uint64_t s = s0; s < s1; ++s)
+    {
+      AHIP(c, copy_state(c, seg[s - s0].p, c->U));
+      if (s + 1 < s1)
+        AHIP(c, step_ctx(c, dt));
+    }
+    for (uint64_t s = s1; s-- > s0;)
+    {
+      // stage states S0..S3 with their trace records and rhs k1..k4
+      AHIP(c, copy_state(c, S[0].p, seg[s - s0].p));
+      AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys, S[0].p, T[0].p, c->status,
+                             c->stream));
+      for (int i = 0; i < 3; ++i)
+        AHIP(c, stage(c, c->phys, S[i].p, T[i].p, S[0].p, S[i + 1].p, T[i + 1].p, kf[i].p,
+                      cstage[i]));
+      AHIP(c, stage(c, c->phys, S[3].p, T[3].p, nullptr, nullptr, nullptr, kf[3].p, 0.0));
+      // reverse: l_(i+1) weights of k_(i+1), a_i adjoint through S_i
+      AHIP(c, copy_state(c, asum.p, lam.p));
+      for (int i = 3; i >= 0; --i)
+      {
+        // lk = w_l l + w_a a_(i+1) (a_(i+1) in acur)
+        const double wl = (i == 3 || i == 0) ? dt / 6.0 : dt / 3.0;
+        const double wa = (i == 2) ? dt : 0.5 * dt;
+        AHIP(c, copy_state(c, lk.p, lam.p));
+        if (i == 3)
+          AHIP(c, axpby_dispatch(lk.p, lk.p, 0.0, wl, n, c->stream));
+        else
+          AHIP(c, axpby_dispatch(lk.p, acur.p, wa, wl, n, c->stream));
+        if (dJdmu)
+        {
+          // df/dmu at S_i: (k_(i+1) - f(S_i; 0)) / mu
+          AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), ph0, S[i].p, tz.p, c->status,
+                                 c->stream));
+          AHIP(c, stage(c, ph0, S[i].p, tz.p, nullptr, nullptr, nullptr, f0.p, 0.0));
+          AHIP(c, axpby_dispatch(f0.p, kf[i].p, 1.0 / c->phys.mu, -1.0 / c->phys.mu, n,
+                                 c->stream));
+          AHIP(c, dot_dispatch(lk.p, f0.p, n, dpart.p, ndot, dsum.p, c->stream));
+          double d = 0.0;
+          AHIP(c, hipMemcpyAsync(&d, dsum.p, sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+          AHIP(c, hipStreamSynchronize(c->stream));
+          dmu_parts.push_back(d);
+        }
+        AHIP(c, adjoint_dispatch(c->p, c->q, mesh_args(c), c->phys, S[i].p, T[i].p, c->lam,
+                                 lk.p, scr.pvol.p, scr.aown.p, scr.anbr.p, acur.p, c->stream));
+        AHIP(c, axpby_dispatch(asum.p, acur.p, 1.0, 1.0, n, c->stream));
+      }
+      AHIP(c, copy_state(c, lam.p, asum.p));
+    }
+  }
+  for (size_t i = dmu_parts.size(); i-- > 0;)
+    dmu_total += dmu_parts[i];
+  // gradient with respect to the initial state in the reference layout
+  if (dJdU0_host)
+  {
+    AHIP(c, basis_dispatch(c->p, lam.p, asum.p, c->nelem, 2, c->stream));
+    AHIP(c, hipMemcpyAsync(dJdU0_host, asum.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  }
+  // the context ends at the final state, as after dgx_step
+  AHIP(c, copy_state(c, c->U, final_state.p));
+  c->tu_valid = false;
+  AHIP(c, hipStreamSynchronize(c->stream));
+  int h = 0;
+  AHIP(c, hipMemcpy(&h, c->status, sizeof(int), hipMemcpyDeviceToHost));
+  if (h)
+  {
+    AHIP(c, hipMemset(c->status, 0, sizeof(int)));
+    return afail(c, DGX_ERR_INVALID, "negative inflow boundary Mach number");
+  }
+  if (J)
+    *J = Jv;
+  if (dJdmu)
+    *dJdmu = dmu_total;
+  return DGX_OK;
+}
+
+}  // extern "C"

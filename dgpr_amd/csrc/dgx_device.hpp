@@ -52,15 +52,15 @@ __device__ __forceinline__ void euler_flux(const double* U, double gamma,
 }
 
 // Normal Euler flux F(U).n
-__device__ __forceinline__ void euler_flux_n(const double* U, double gamma,
-                                             const double* n, double* Fn)
+template <typename T>
+__device__ __forceinline__ void euler_flux_n(const T* U, double gamma, const double* n, T* Fn)
 {
-  double rho = U[0];
-  double ir  = rcp(rho);
-  double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
-  double p  = (gamma - 1.0) * (U[4] - 0.5 * rho * (u * u + v * v + w * w));
-  double un = u * n[0] + v * n[1] + w * n[2];
-  double mn = rho * un;
+  T rho = U[0];
+  T ir  = rcp(rho);
+  T u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
+  T p  = (gamma - 1.0) * (U[4] - 0.5 * rho * (u * u + v * v + w * w));
+  T un = u * n[0] + v * n[1] + w * n[2];
+  T mn = rho * un;
   Fn[0] = mn;
   Fn[1] = mn * u + p * n[0];
   Fn[2] = mn * v + p * n[1];
@@ -69,45 +69,46 @@ __device__ __forceinline__ void euler_flux_n(const double* U, double gamma,
 }
 
 // Roe flux with |lambda| entropy fix below 5% of the Roe sound speed.
-__device__ __forceinline__ void roe_flux(const double* Ul, const double* Ur,
-                                         const double* n, double g, double* f)
+template <typename T>
+__device__ __forceinline__ void roe_flux(const T* Ul, const T* Ur, const double* n, double g,
+                                         T* f)
 {
   double gm1 = g - 1.0;
-  double irl = rcp(Ul[0]), irr = rcp(Ur[0]);
-  double ul = Ul[1] * irl, vl = Ul[2] * irl, wl = Ul[3] * irl;
-  double ur = Ur[1] * irr, vr = Ur[2] * irr, wr = Ur[3] * irr;
-  double pl = gm1 * (Ul[4] - 0.5 * Ul[0] * (ul * ul + vl * vl + wl * wl));
-  double pr = gm1 * (Ur[4] - 0.5 * Ur[0] * (ur * ur + vr * vr + wr * wr));
-  double Hl = (Ul[4] + pl) * irl;
-  double Hr = (Ur[4] + pr) * irr;
-  double sl = sqrt(Ul[0]), sr = sqrt(Ur[0]);
-  double iden = rcp(sl + sr);
-  double u = (sl * ul + sr * ur) * iden;
-  double v = (sl * vl + sr * vr) * iden;
-  double w = (sl * wl + sr * wr) * iden;
-  double H = (sl * Hl + sr * Hr) * iden;
-  double dr = Ur[0] - Ul[0], dxm = Ur[1] - Ul[1], dym = Ur[2] - Ul[2];
-  double dzm = Ur[3] - Ul[3], drE = Ur[4] - Ul[4];
-  double qsq = u * u + v * v + w * w;
-  double c   = sqrt(gm1 * (H - 0.5 * qsq));
-  double ic  = rcp(c);
-  double un  = u * n[0] + v * n[1] + w * n[2];
-  double l1 = un + c, l2 = un - c, l3 = un;
-  double al1 = fabs(l1), al2 = fabs(l2), al3 = fabs(l3);
-  double eps  = 0.05 * c;
-  double i2e  = rcp(2.0 * eps);
+  T irl = rcp(Ul[0]), irr = rcp(Ur[0]);
+  T ul = Ul[1] * irl, vl = Ul[2] * irl, wl = Ul[3] * irl;
+  T ur = Ur[1] * irr, vr = Ur[2] * irr, wr = Ur[3] * irr;
+  T pl = gm1 * (Ul[4] - 0.5 * Ul[0] * (ul * ul + vl * vl + wl * wl));
+  T pr = gm1 * (Ur[4] - 0.5 * Ur[0] * (ur * ur + vr * vr + wr * wr));
+  T Hl = (Ul[4] + pl) * irl;
+  T Hr = (Ur[4] + pr) * irr;
+  T sl = sqrt(Ul[0]), sr = sqrt(Ur[0]);
+  T iden = rcp(sl + sr);
+  T u = (sl * ul + sr * ur) * iden;
+  T v = (sl * vl + sr * vr) * iden;
+  T w = (sl * wl + sr * wr) * iden;
+  T H = (sl * Hl + sr * Hr) * iden;
+  T dr = Ur[0] - Ul[0], dxm = Ur[1] - Ul[1], dym = Ur[2] - Ul[2];
+  T dzm = Ur[3] - Ul[3], drE = Ur[4] - Ul[4];
+  T qsq = u * u + v * v + w * w;
+  T c   = sqrt(gm1 * (H - 0.5 * qsq));
+  T ic  = rcp(c);
+  T un  = u * n[0] + v * n[1] + w * n[2];
+  T l1 = un + c, l2 = un - c, l3 = un;
+  T al1 = fabs(l1), al2 = fabs(l2), al3 = fabs(l3);
+  T eps  = 0.05 * c;
+  T i2e  = rcp(2.0 * eps);
   if (al1 < eps) al1 = fabs((eps * eps + l1 * l1) * i2e);
   if (al2 < eps) al2 = fabs((eps * eps + l2 * l2) * i2e);
   if (al3 < eps) al3 = fabs((eps * eps + l3 * l3) * i2e);
-  double s1 = 0.5 * (al1 + al2);
-  double s2 = 0.5 * (al1 - al2);
-  double G1 = gm1 * (0.5 * qsq * dr - (u * dxm + v * dym + w * dzm) + drE);
-  double G2 = -un * dr + (dxm * n[0] + dym * n[1] + dzm * n[2]);
-  double C1 = (G1 * ic * ic) * (s1 - al3) + (G2 * ic) * s2;
-  double C2 = (G1 * ic) * s2 + (s1 - al3) * G2;
-  double unl = ul * n[0] + vl * n[1] + wl * n[2];
-  double unr = ur * n[0] + vr * n[1] + wr * n[2];
-  double mnl = Ul[0] * unl, mnr = Ur[0] * unr;
+  T s1 = 0.5 * (al1 + al2);
+  T s2 = 0.5 * (al1 - al2);
+  T G1 = gm1 * (0.5 * qsq * dr - (u * dxm + v * dym + w * dzm) + drE);
+  T G2 = -un * dr + (dxm * n[0] + dym * n[1] + dzm * n[2]);
+  T C1 = (G1 * ic * ic) * (s1 - al3) + (G2 * ic) * s2;
+  T C2 = (G1 * ic) * s2 + (s1 - al3) * G2;
+  T unl = ul * n[0] + vl * n[1] + wl * n[2];
+  T unr = ur * n[0] + vr * n[1] + wr * n[2];
+  T mnl = Ul[0] * unl, mnr = Ur[0] * unr;
   f[0] = 0.5 * (mnl + mnr) - 0.5 * (al3 * dr + C1);
   f[1] = 0.5 * ((mnl * ul + pl * n[0]) + (mnr * ur + pr * n[0])) -
          0.5 * (al3 * dxm + C1 * u + C2 * n[0]);

@@ -1,0 +1,164 @@
+// dgx_dual.hpp -- forward-mode dual numbers for the local Jacobians of the
+// discrete adjoint: a value with D directional derivatives.  The pointwise
+// physics (dgx_physics.hpp) is written for a generic scalar type, so running
+// it on Dual<D> inputs seeded with unit directions gives D columns of its
+// Jacobian at once.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dgx_device.hpp"
+
+namespace dgx
+{
+
+// the dual overloads below must not hide the double ones
+using ::fabs;
+using ::sqrt;
+
+template <int D>
+struct Dual
+{
+  double v;
+  double d[D];
+  __device__ Dual() {}
+  __device__ Dual(double x) : v(x)
+  {
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      d[k] = 0.0;
+  }
+};
+
+template <int D>
+__device__ __forceinline__ Dual<D> operator+(const Dual<D>& a, const Dual<D>& b)
+{
+  Dual<D> r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator-(const Dual<D>& a, const Dual<D>& b)
+{
+  Dual<D> r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator-(const Dual<D>& a)
+{
+  Dual<D> r;
+  r.v = -a.v;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = -a.d[k];
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator*(const Dual<D>& a, const Dual<D>& b)
+{
+  Dual<D> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator*(double a, const Dual<D>& b)
+{
+  Dual<D> r;
+  r.v = a * b.v;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = a * b.d[k];
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator*(const Dual<D>& b, double a)
+{
+  return a * b;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator+(const Dual<D>& a, double b)
+{
+  Dual<D> r = a;
+  r.v += b;
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator+(double b, const Dual<D>& a)
+{
+  return a + b;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator-(const Dual<D>& a, double b)
+{
+  Dual<D> r = a;
+  r.v -= b;
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator-(double b, const Dual<D>& a)
+{
+  return (-a) + b;
+}
+template <int D>
+__device__ __forceinline__ Dual<D>& operator+=(Dual<D>& a, const Dual<D>& b)
+{
+  a = a + b;
+  return a;
+}
+template <int D>
+__device__ __forceinline__ Dual<D>& operator-=(Dual<D>& a, const Dual<D>& b)
+{
+  a = a - b;
+  return a;
+}
+template <int D>
+__device__ __forceinline__ Dual<D>& operator*=(Dual<D>& a, double b)
+{
+  a = b * a;
+  return a;
+}
+template <int D>
+__device__ __forceinline__ bool operator<(const Dual<D>& a, const Dual<D>& b)
+{
+  return a.v < b.v;
+}
+// 1/x, with the same refined reciprocal as the residual kernels for the value
+template <int D>
+__device__ __forceinline__ Dual<D> rcp(const Dual<D>& x)
+{
+  Dual<D> r;
+  r.v = rcp(x.v);
+  const double s = -r.v * r.v;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = s * x.d[k];
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> sqrt(const Dual<D>& x)
+{
+  Dual<D> r;
+  r.v = ::sqrt(x.v);
+  const double s = 0.5 / r.v;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = s * x.d[k];
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> fabs(const Dual<D>& x)
+{
+  return (x.v < 0.0) ? -x : x;
+}
+
+}  // namespace dgx

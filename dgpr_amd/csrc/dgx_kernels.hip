@@ -28,7 +28,8 @@
 // neighbour's gradient.
 #include <hip/hip_runtime.h>
 
-#include "dgx_device.hpp"
+#include "dgx_dual.hpp"
+#include "dgx_physics.hpp"
 
 namespace dgx
 {
@@ -283,159 +284,6 @@ struct FaceDims
   static constexpr int RG    = (5 * NB * EPW > FS * 64) ? 5 * NB * EPW : FS * 64;
 };
 
-// ---- viscous flux in velocity-gradient form ---------------------------------
-// The Newtonian flux A(s, grad s) of flux.cpp:156-202 depends on the
-// conserved-variable gradient only through the velocity gradient
-//   Lv[i][d] = d u_i / d x_d = (d m_i/dx_d - u_i d rho/dx_d) / rho
-// (there is no heat flux), and every BR2 / dual-consistency argument is a
-// jump times the normal, whose velocity gradient is rank one:
-//   (U_jump (x) n)  ->  dm (x) n,   dm_i = (jump_m_i - u_i jump_rho) / rho.
-// tau(X) = mu (X + X^T) - 2/3 mu tr(X) I;  the energy row is u . tau.
-
-// w = tau(X) n for X with X n = xn, X^T n = xtn, tr X = tr
-__device__ __forceinline__ void tau_n(double mu, const double* xn, const double* xtn, double tr,
-                                      const double* n, double* w)
-{
-  double lt = -(2.0 / 3.0) * mu * tr;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    w[i] = mu * (xn[i] + xtn[i]) + lt * n[i];
-}
-
-// velocity gradient from reference-space derivatives gr[c][0..3] of
-// (rho, m) and the inverse Jacobian Ji[c][d]
-__device__ __forceinline__ void vel_grad(const double gr[3][4], const double* u, double ir,
-                                         const double Ji[3][3], double Lv[3][3])
-{
-  double dv[3][3];  // dv[c][i] = d u_i / d xi_c
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-      dv[c][i] = (gr[c][1 + i] - gr[c][0] * u[i]) * ir;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-      Lv[i][d] = dv[0][i] * Ji[0][d] + dv[1][i] * Ji[1][d] + dv[2][i] * Ji[2][d];
-}
-
-// One face point seen from the own side: outputs G = F - QH.n (5) and the
-// physical dual-consistency flux dcQ as a symmetric stress S (6: xx yy zz
-// xy xz yz) whose energy row is u_L . S.
-//
-// Interior (solve.cpp:393-626):
-//   F   = roe(UL, UR, n)
-//   dcQ = A(UL, (UL - (UL+UR)/2) n)            = tau(dmL/2 (x) n)
-//   QH  = (A(UL,gL) + A(UR,gR))/2 - eta (dL + dR)/2,
-//   dL  = lamL/2 A(UL, (UL-UR) n),  dR = lamR/2 A(UR, (UL-UR) n)
-// so QH.n = tau(Lv_L/2 - cL dmL(x)n) n + tau(Lv_R/2 - cR dmR(x)n) n with
-// c = eta lam / 4, plus the energy rows u_L.(...) and u_R.(...).
-// Boundary (solve.cpp:628-794): Ub = boundary state, F = F(Ub).n,
-//   dcQ = tau(dm (x) n) with jump UL - Ub,  QH.n = A(UL, gL).n - eta lamL A(Ub, (UL-Ub) n).n
-struct FacePoint
-{
-  double G[5];
-  double S[6];
-};
-
-// Gradient traction tau(grad u).n of one side, from the reference-space
-// derivatives gr[c][0..3] of (rho, m), the inverse Jacobian Ji and the normal:
-// with dv[c][i] = d u_i/d xi_c,  (L n)_i = sum_c dv[c][i] (Ji n)_c,
-// (L^T n)_d = sum_c Ji[c][d] (dv[c] . n),  tr L = sum_c dv[c] . Ji[c].
-__device__ __forceinline__ void grad_traction(const double gr[3][4], const double* U,
-                                              const double Ji[3][3], const double* n, double mu,
-                                              double* vg)
-{
-  double ir = rcp(U[0]);
-  double u[3] = {U[1] * ir, U[2] * ir, U[3] * ir};
-  double Ln[3] = {0.0, 0.0, 0.0}, LTn[3] = {0.0, 0.0, 0.0}, tr = 0.0;
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-  {
-    double dv0 = (gr[c][1] - gr[c][0] * u[0]) * ir;
-    double dv1 = (gr[c][2] - gr[c][0] * u[1]) * ir;
-    double dv2 = (gr[c][3] - gr[c][0] * u[2]) * ir;
-    double mc = Ji[c][0] * n[0] + Ji[c][1] * n[1] + Ji[c][2] * n[2];
-    double kc = dv0 * n[0] + dv1 * n[1] + dv2 * n[2];
-    Ln[0] += dv0 * mc;
-    Ln[1] += dv1 * mc;
-    Ln[2] += dv2 * mc;
-    LTn[0] += Ji[c][0] * kc;
-    LTn[1] += Ji[c][1] * kc;
-    LTn[2] += Ji[c][2] * kc;
-    tr += dv0 * Ji[c][0] + dv1 * Ji[c][1] + dv2 * Ji[c][2];
-  }
-  tau_n(mu, Ln, LTn, tr, n, vg);
-}
-
-// vgL, vgR: gradient tractions tau(grad u).n of the own and neighbour traces
-// (vgR unused on boundary faces).  tau(a (x) n).n = mu a + mu/3 (a.n) n.
-// Ub is the neighbour trace (interior) or the boundary state (boundary).
-__device__ __forceinline__ void face_point_physics(bool interior, const double* UL,
-                                                   const double* Ub, const double* vgL,
-                                                   const double* vgR, const double* n,
-                                                   double lamL, double lamR, const Phys& ph,
-                                                   FacePoint& out)
-{
-  const double mu = ph.mu;
-  double irL = rcp(UL[0]);
-  double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
-  if (interior)
-    roe_flux(UL, Ub, n, ph.gamma, out.G);
-  else
-    euler_flux_n(Ub, ph.gamma, n, out.G);
-  double irB = rcp(Ub[0]);
-  double uB[3] = {Ub[1] * irB, Ub[2] * irB, Ub[3] * irB};
-  double jr = UL[0] - Ub[0];
-  double dmL[3], dmB[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-  {
-    double jm = UL[1 + i] - Ub[1 + i];
-    dmL[i] = (jm - jr * uL[i]) * irL;
-    dmB[i] = (jm - jr * uB[i]) * irB;
-  }
-  double dnL = dmL[0] * n[0] + dmL[1] * n[1] + dmL[2] * n[2];
-  double dnB = dmB[0] * n[0] + dmB[1] * n[1] + dmB[2] * n[2];
-
-  // own side: hL tau(grad u_L).n - cL tau(dmL (x) n).n (interior: halved,
-  // with the penalty; boundary: the full gradient traction)
-  double hL = interior ? 0.5 : 1.0;
-  double cL = interior ? 0.25 * ph.eta * lamL : 0.0;
-  double w[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    w[i] = hL * vgL[i] - cL * mu * (dmL[i] + (1.0 / 3.0) * dnL * n[i]);
-  double q[4] = {w[0], w[1], w[2], uL[0] * w[0] + uL[1] * w[1] + uL[2] * w[2]};
-
-  // other side: neighbour (interior) or boundary penalty at the boundary state
-  double cB = interior ? 0.25 * ph.eta * lamR : ph.eta * lamL;
-  double hB = interior ? 0.5 : 0.0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    w[i] = hB * vgR[i] - cB * mu * (dmB[i] + (1.0 / 3.0) * dnB * n[i]);
-  q[0] += w[0];
-  q[1] += w[1];
-  q[2] += w[2];
-  q[3] += uB[0] * w[0] + uB[1] * w[1] + uB[2] * w[2];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    out.G[1 + i] -= q[i];
-  out.G[4] -= q[3];
-
-  // dual-consistency stress tau(s dmL (x) n), s = 1/2 (interior) or 1
-  double s  = interior ? 0.5 : 1.0;
-  double a0 = s * dmL[0], a1 = s * dmL[1], a2 = s * dmL[2];
-  double lt = -(2.0 / 3.0) * mu * s * dnL;
-  out.S[0] = 2.0 * mu * a0 * n[0] + lt;
-  out.S[1] = 2.0 * mu * a1 * n[1] + lt;
-  out.S[2] = 2.0 * mu * a2 * n[2] + lt;
-  out.S[3] = mu * (a0 * n[1] + a1 * n[0]);
-  out.S[4] = mu * (a0 * n[2] + a2 * n[0]);
-  out.S[5] = mu * (a1 * n[2] + a2 * n[1]);
-}
-
 // Face trace records of the EPW elements whose Gauss-nodal states are in sV:
 // for each local face f and face point, the trace U (5) and the gradient
 // traction tau(grad u).n (3) taken with the element's own outward unit
@@ -667,27 +515,7 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
       gr[1][r] = g1;
       gr[2][r] = g2;
     }
-    double ir = rcp(Vq[0]);
-    double u[3] = {Vq[1] * ir, Vq[2] * ir, Vq[3] * ir};
-    double Lv[3][3];
-    vel_grad(gr, u, ir, Ji, Lv);
-    double p = (ph.gamma - 1.0) * (Vq[4] - 0.5 * Vq[0] * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-    double Ht = (Vq[4] + p) * ir;
-    double lt = -(2.0 / 3.0) * ph.mu * (Lv[0][0] + Lv[1][1] + Lv[2][2]);
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-    {
-      double tau[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        tau[i] = ph.mu * (Lv[d][i] + Lv[i][d]) + ((i == d) ? lt : 0.0);
-      double md = Vq[1 + d];
-      H[d][0] = -md;
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        H[d][1 + i] = tau[i] - md * u[i] - ((i == d) ? p : 0.0);
-      H[d][4] = u[0] * tau[0] + u[1] * tau[1] + u[2] * tau[2] - md * Ht;
-    }
+    volume_flux(Vq, gr, Ji, ph.gamma, ph.mu, H);
   }
 
   // ---- volume divergence, one reference direction at a time ----
@@ -1061,6 +889,8 @@ __device__ __forceinline__ DD dd_fma(DD x, double c, double clo, DD acc)
 
 // mode 0: V = B3 U      (reference basis -> Gauss-nodal)
 // mode 1: U = B3^-1 V   (Gauss-nodal -> reference basis)
+// mode 2: G = B3^T g    (a gradient with respect to V -> with respect to U)
+// mode 3: g = B3^-T G   (a gradient with respect to U -> with respect to V)
 template <int P>
 __global__ __launch_bounds__(64) void dgx_basis_kernel(const double* in, double* out, int nelem,
                                                        int mode)
@@ -1080,8 +910,10 @@ __global__ __launch_bounds__(64) void dgx_basis_kernel(const double* in, double*
       sl[le][r][node] = 0.0;
     }
   __syncthreads();
-  const double(*M)[MAXN]  = mode == 0 ? pt.B : pt.Binv;
-  const double(*Ml)[MAXN] = mode == 0 ? pt.Blo : pt.Binvlo;
+  const bool inv = (mode == 1 || mode == 3);
+  const double(*M)[MAXN]  = inv ? pt.Binv : pt.B;
+  const double(*Ml)[MAXN] = inv ? pt.Binvlo : pt.Blo;
+  const bool tr = (mode >= 2);
   int idx[3] = {node % N, (node / N) % N, node / (N * N)};
 #pragma unroll
   for (int d = 0; d < 3; ++d)
@@ -1099,7 +931,9 @@ __global__ __launch_bounds__(64) void dgx_basis_kernel(const double* in, double*
           int id2[3] = {idx[0], idx[1], idx[2]};
           id2[d] = m;
           int src = (id2[2] * N + id2[1]) * N + id2[0];
-          acc = dd_fma({sh[le][r][src], sl[le][r][src]}, M[idx[d]][m], Ml[idx[d]][m], acc);
+          const double c  = tr ? M[m][idx[d]] : M[idx[d]][m];
+          const double cl = tr ? Ml[m][idx[d]] : Ml[idx[d]][m];
+          acc = dd_fma({sh[le][r][src], sl[le][r][src]}, c, cl, acc);
         }
         o[r] = acc;
       }
@@ -1324,6 +1158,9 @@ __global__ __launch_bounds__(256) void dgx_mean_kernel(const double* __restrict_
     mean[i] = (mean[i] * old_range + U[i]) / new_range;
 }
 
+// ---- discrete adjoint --------------------------------------------------------
+#include "dgx_adjoint_kernels.hpp"
+
 // ---- launch helpers (host) -----------------------------------------------
 
 template <int P, int Q>
@@ -1520,6 +1357,70 @@ hipError_t mean_dispatch(const double* U, double* mean, size_t n, double old_ran
   hipLaunchKernelGGL(dgx_mean_kernel, dim3(nblk), dim3(256), 0, st, U, mean, n, old_range,
                      new_range);
   return hipGetLastError();
+}
+
+template <int P, int Q>
+hipError_t launch_adjoint(const MeshArgs& m, const Phys& ph, const double* V, const double* Tin,
+                          const double* lam, const double* psi, double* pvol, double* aown,
+                          double* anbr, double* psiV, hipStream_t st)
+{
+  constexpr int EPW = Dims<P, Q>::EPW;
+  int nblk = (m.nelem + EPW - 1) / EPW;
+  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, V, Tin,
+                     lam, psi, pvol, aown, anbr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  hipLaunchKernelGGL((dgx_adj_trace_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, V, aown,
+                     anbr, pvol, psiV);
+  return hipGetLastError();
+}
+
+hipError_t adjoint_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
+                            const double* Tin, const double* lam, const double* psi,
+                            double* pvol, double* aown, double* anbr, double* psiV,
+                            hipStream_t st)
+{
+#define CALL_ADJ(P, Q) launch_adjoint<P, Q>(m, ph, V, Tin, lam, psi, pvol, aown, anbr, psiV, st)
+  DGX_DISPATCH_PQ(p, q, CALL_ADJ)
+#undef CALL_ADJ
+}
+
+template <int P, int Q>
+hipError_t launch_kinetic_energy(const MeshArgs& m, const double* V, double* grad,
+                                 double* partial, hipStream_t st)
+{
+  constexpr int EPW = Dims<P, Q>::EPW;
+  int nblk = (m.nelem + EPW - 1) / EPW;
+  hipLaunchKernelGGL((dgx_kinetic_energy_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, V, grad,
+                     partial);
+  return hipGetLastError();
+}
+
+hipError_t kinetic_energy_dispatch(int p, int q, const MeshArgs& m, const double* V,
+                                   double* grad, double* partial, hipStream_t st)
+{
+#define CALL_KE(P, Q) launch_kinetic_energy<P, Q>(m, V, grad, partial, st)
+  DGX_DISPATCH_PQ(p, q, CALL_KE)
+#undef CALL_KE
+}
+
+hipError_t axpby_dispatch(double* y, const double* x, double a, double b, size_t n,
+                          hipStream_t st)
+{
+  int nblk = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(dgx_axpby_kernel, dim3(nblk), dim3(256), 0, st, y, x, a, b, n);
+  return hipGetLastError();
+}
+
+hipError_t dot_dispatch(const double* x, const double* y, size_t n, double* partial,
+                        int nblocks, double* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(dgx_dot_kernel, dim3(nblocks), dim3(256), 0, st, x, y, n, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  return sum_dispatch(partial, nblocks, out, st);
 }
 
 hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st)

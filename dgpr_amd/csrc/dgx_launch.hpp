@@ -108,6 +108,22 @@ hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg,
 hipError_t basis_dispatch(int p, const double* in, double* out, int nelem, int mode,
                           hipStream_t st);
 hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st);
+// psiV = (d f_g / d V)^T psi on a periodic mesh, V with its trace records Tin;
+// pvol, aown, anbr are scratch (a state and two trace-record sets)
+hipError_t adjoint_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
+                            const double* Tin, const double* lam, const double* psi,
+                            double* pvol, double* aown, double* anbr, double* psiV,
+                            hipStream_t st);
+// kinetic energy (per-block partials in partial[resid_out_blocks]) and its
+// gradient with respect to the Gauss-nodal state
+hipError_t kinetic_energy_dispatch(int p, int q, const MeshArgs& m, const double* V,
+                                   double* grad, double* partial, hipStream_t st);
+// y = a x + b y
+hipError_t axpby_dispatch(double* y, const double* x, double a, double b, size_t n,
+                          hipStream_t st);
+// *out = x . y (nblocks partial sums in partial)
+hipError_t dot_dispatch(const double* x, const double* y, size_t n, double* partial,
+                        int nblocks, double* out, hipStream_t st);
 // derived outputs of a Gauss-nodal state on the (Ns+1)^3 sub-grid of each element
 hipError_t sample_dispatch(int p, const double* V, int nelem, int Ns, int var, double gamma,
                            double* out, hipStream_t st);

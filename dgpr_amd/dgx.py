@@ -109,6 +109,9 @@ def lib():
         L.dgx_read_dg_header.argtypes = [C.c_char_p, C.POINTER(U64), C.POINTER(_CMesh)]
         L.dgx_read_dg.argtypes = [C.c_char_p, PD, PD]
         L.dgx_set_tuning.argtypes = [P_ctx, C.c_int]
+        L.dgx_kinetic_energy.argtypes = [P_ctx, PD]
+        L.dgx_residual_vjp.argtypes = [P_ctx, PD, PD]
+        L.dgx_adjoint_rk4.argtypes = [P_ctx, D, U64, U64, PD, PD, PD]
         L.dgx_ensemble_create.argtypes = [C.c_int, C.POINTER(_CMesh), C.POINTER(Params),
                                           C.POINTER(C.c_void_p)]
         L.dgx_ensemble_destroy.argtypes = [C.c_void_p]
@@ -136,7 +139,8 @@ def lib():
                    "dgx_sample_output_device", "dgx_update_mean", "dgx_get_mean",
                    "dgx_ensemble_create", "dgx_ensemble_destroy", "dgx_ensemble_member",
                    "dgx_ensemble_step", "dgx_ensemble_sync", "dgx_ensemble_get_stream",
-                   "dgx_ensemble_autotune"):
+                   "dgx_ensemble_autotune", "dgx_kinetic_energy", "dgx_residual_vjp",
+                   "dgx_adjoint_rk4"):
             getattr(L, nm).restype = st
         _lib = L
     return _lib
@@ -341,6 +345,29 @@ class Solver:
     @stage_waves.setter
     def stage_waves(self, w):
         _chk(lib().dgx_set_tuning(self.ctx, int(w)), self.ctx)
+
+    def kinetic_energy(self):
+        """Volume integral of |m|^2 / (2 rho) of the current state."""
+        J = C.c_double()
+        _chk(lib().dgx_kinetic_energy(self.ctx, C.byref(J)), self.ctx)
+        return J.value
+
+    def residual_vjp(self, w):
+        """(df/dU)^T w at the current state (periodic meshes)."""
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+        g = np.zeros(self.nstate)
+        _chk(lib().dgx_residual_vjp(self.ctx, _p(w), _p(g)), self.ctx)
+        return g
+
+    def adjoint_rk4(self, dt, nsteps, interval=0, want_mu=True):
+        """Final kinetic energy after nsteps RK4 steps and its gradients with
+        respect to the initial state and to mu (discrete adjoint)."""
+        J = C.c_double()
+        g = np.zeros(self.nstate)
+        dmu = C.c_double()
+        _chk(lib().dgx_adjoint_rk4(self.ctx, dt, int(nsteps), int(interval), C.byref(J), _p(g),
+                                   C.byref(dmu) if want_mu else None), self.ctx)
+        return J.value, g, (dmu.value if want_mu else None)
 
     def sample_output(self, var, N):
         """eval_output (render/vrtxgen.cpp:25-59) of the current state on the

@@ -232,6 +232,26 @@ dgx_status dgx_ensemble_autotune(dgx_ensemble* ens, int scheme, const double* dt
                                  int* waves_out);
 const char* dgx_ensemble_last_error(const dgx_ensemble* ens);
 
+/* ---- discrete adjoint (no counterpart in the reference) --------------------
+ * Exact transposes of the device's discrete residual and RK4 stages, for
+ * periodic meshes (DGX_ERR_UNSUPPORTED otherwise). */
+
+/* J = volume integral of the kinetic energy |m|^2 / (2 rho) of the context's
+ * state, with the (p+1)^3 Gauss rule of the mass matrix. */
+dgx_status dgx_kinetic_energy(dgx_ctx* ctx, double* J);
+
+/* g = (df/dU)^T w at the context's state: the vector-Jacobian product of
+ * residual()'s f = -Minv R (solve.cpp:150-807); w and g in simstate layout. */
+dgx_status dgx_residual_vjp(dgx_ctx* ctx, const double* w_host, double* g_host);
+
+/* Runs nsteps classic RK4 steps of size dt from the context's state (which
+ * ends at the final state, as after dgx_step) and returns the final kinetic
+ * energy J, its gradient with respect to the initial state (simstate layout,
+ * may be NULL) and dJ/dmu (may be NULL), by the discrete adjoint with a
+ * checkpoint every `interval` steps (0: ceil(sqrt(nsteps))). */
+dgx_status dgx_adjoint_rk4(dgx_ctx* ctx, double dt, uint64_t nsteps, uint64_t interval,
+                           double* J, double* dJdU0_host, double* dJdmu);
+
 /* ---- launch tuning (no counterpart in the reference, whose kernels have
  * fixed launch shapes, solve_cuda.cu:1235-1282) ---------------------------- */
 

@@ -1,0 +1,105 @@
+"""Discrete adjoint against central finite differences of the forward path:
+  * residual level: w . (df/dU v) by central differences of dgx_residual
+    equals v . (df/dU)^T w from dgx_residual_vjp (dot-product test of the
+    tangent-linear residual against its adjoint);
+  * time stepping: the gradient of the final kinetic energy after N RK4
+    steps with respect to the initial state (random directions) and to mu,
+    from dgx_adjoint_rk4 with checkpointing, equals central differences of
+    dgx_step + dgx_kinetic_energy.
+Central differences carry O(eps^2) truncation and O(1e-16 |f| / eps)
+rounding; the tolerances below (1e-6 relative) sit well above both for the
+chosen eps and well below any missing or mis-signed term."""
+import numpy as np
+import pytest
+
+from dgpr_amd import cases, dgx
+
+pytestmark = pytest.mark.gpu
+
+FD_TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def built():
+    dgx.lib()
+    return True
+
+
+def small_cases():
+    return {
+        "tgv2_p2_curved": cases.tgv_case(n=2, p=2, seed=51, M=0.3, Re=400.0, amp=0.15, dt=2e-3),
+        "tgv2_p3": cases.tgv_case(n=2, p=3, seed=52, M=0.25, Re=800.0, dt=1.5e-3),
+        "tgv3_p1": cases.tgv_case(n=3, p=1, seed=53, M=0.3, Re=400.0, dt=4e-3),
+    }
+
+
+def rng_dir(U, seed):
+    rng = np.random.default_rng(seed)
+    return U * rng.uniform(-1.0, 1.0, U.shape)
+
+
+@pytest.mark.parametrize("name", ["tgv2_p2_curved", "tgv2_p3", "tgv3_p1"])
+def test_residual_vjp_dot_product(built, name):
+    c = small_cases()[name]
+    U = c.U0
+    v = rng_dir(U, 1)
+    w = np.random.default_rng(2).uniform(-1.0, 1.0, U.shape)
+    eps = 1e-6
+    with dgx.Solver(c.mesh, c.params) as s:
+        s.set_state(U + eps * v)
+        _, fp = s.residual()
+        s.set_state(U - eps * v)
+        _, fm = s.residual()
+        s.set_state(U)
+        g = s.residual_vjp(w)
+    lhs = float(w @ ((fp - fm) / (2 * eps)))
+    rhs = float(v @ g)
+    err = abs(lhs - rhs) / abs(rhs)
+    print("  %s: w.Jv (FD) %.12e  v.J^T w %.12e  rel %.2e" % (name, lhs, rhs, err), flush=True)
+    assert err < FD_TOL
+
+
+def final_energy(c, U0, nsteps, mu=None):
+    prm = c.params.copy()
+    if mu is not None:
+        prm.mu = mu
+    with dgx.Solver(c.mesh, prm) as s:
+        s.set_state(U0)
+        s.step(c.dt, "RK4", nsteps, norms=False)
+        return s.kinetic_energy()
+
+
+@pytest.mark.parametrize("name,nsteps,interval", [("tgv2_p2_curved", 6, 0), ("tgv3_p1", 5, 2),
+                                                  ("tgv2_p3", 4, 3)])
+def test_adjoint_rk4_gradient_matches_fd(built, name, nsteps, interval):
+    c = small_cases()[name]
+    with dgx.Solver(c.mesh, c.params) as s:
+        s.set_state(c.U0)
+        J, g, dmu = s.adjoint_rk4(c.dt, nsteps, interval)
+        assert abs(J - final_energy(c, c.U0, nsteps)) <= 1e-14 * abs(J)
+    eps = 1e-6
+    for k in range(3):
+        d = rng_dir(c.U0, 10 + k)
+        fd = (final_energy(c, c.U0 + eps * d, nsteps) - final_energy(c, c.U0 - eps * d, nsteps)) / (2 * eps)
+        ad = float(g @ d)
+        err = abs(fd - ad) / abs(ad)
+        print("  %s dir %d: FD %.12e  adjoint %.12e  rel %.2e" % (name, k, fd, ad, err), flush=True)
+        assert err < FD_TOL
+    mu = c.params.mu
+    dm = 1e-4 * mu
+    fd = (final_energy(c, c.U0, nsteps, mu + dm) - final_energy(c, c.U0, nsteps, mu - dm)) / (2 * dm)
+    err = abs(fd - dmu) / abs(dmu)
+    print("  %s dJ/dmu: FD %.12e  adjoint %.12e  rel %.2e" % (name, fd, dmu, err), flush=True)
+    assert err < FD_TOL
+
+
+def test_adjoint_needs_periodic_mesh(built):
+    c = cases.channel_case(3, 3, 2, p=2, seed=16)
+    with dgx.Solver(c.mesh, c.params) as s:
+        s.set_state(c.U0)
+        with pytest.raises(dgx.DgxError) as ei:
+            s.adjoint_rk4(c.dt, 2)
+        assert ei.value.status == dgx.ERR_UNSUPPORTED
+        with pytest.raises(dgx.DgxError) as ei:
+            s.residual_vjp(np.ones(c.mesh.state_size))
+        assert ei.value.status == dgx.ERR_UNSUPPORTED
