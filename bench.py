@@ -175,13 +175,116 @@ def run_ensemble(a, h):
     print(json.dumps(out), flush=True)
 
 
+def run_adjoint(a, h):
+    """Discrete adjoint on the 32^3 p=2 periodic random-smooth-field case:
+    forward time of N RK4 steps, time of dgx_adjoint_rk4 (forward sweep with
+    checkpoints every ceil(sqrt(N)) steps, one recomputation, reverse sweep;
+    gradients with respect to the initial state and to mu), their ratio, and
+    the checks: central differences (eps = 1e-6) of the final kinetic energy
+    along 20 seeded random directions, central differences in mu, and a
+    residual-level dot-product test of the tangent-linear residual against
+    its adjoint.  N is the workload's (100); --steps overrides it."""
+    c = cases.config4()
+    N = a.steps if a.steps_given else c.nsteps
+    U0 = c.U0
+    with dgx.Solver(c.mesh, c.params) as s:
+        s.set_state(U0)
+        s.step_async(c.dt, "RK4", max(a.warmup, 1))
+        s.sync()
+        s.set_state(U0)
+        h.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        s.step_async(c.dt, "RK4", N)
+        s.sync()
+        t_fwd = time.perf_counter() - t0
+        J_fwd = s.kinetic_energy()
+        s.set_state(U0)
+        h.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        J, g, dmu = s.adjoint_rk4(c.dt, N)
+        t_adj = time.perf_counter() - t0
+        s.set_state(U0)
+        t0 = time.perf_counter()
+        s.adjoint_rk4(c.dt, N, want_mu=False)
+        t_adj_state = time.perf_counter() - t0
+
+        def energy(U, mu=None):
+            if mu is not None:
+                prm = c.params.copy()
+                prm.mu = mu
+                s.set_params(prm)
+            s.set_state(U)
+            s.step_async(c.dt, "RK4", N)
+            s.sync()
+            e = s.kinetic_energy()
+            if mu is not None:
+                s.set_params(c.params)
+            return e
+
+        rng = dgx.np.random.default_rng(4)
+        eps, errs = 1e-6, []
+        for k in range(20):
+            d = U0 * rng.uniform(-1.0, 1.0, U0.shape)
+            fd = (energy(U0 + eps * d) - energy(U0 - eps * d)) / (2 * eps)
+            ad = float(g @ d)
+            errs.append(abs(fd - ad) / abs(ad))
+        mu = c.params.mu
+        fdmu = (energy(U0, mu * (1 + 1e-4)) - energy(U0, mu * (1 - 1e-4))) / (2e-4 * mu)
+        # residual-level dot-product test at the initial state
+        v = U0 * rng.uniform(-1.0, 1.0, U0.shape)
+        w = rng.uniform(-1.0, 1.0, U0.shape)
+        s.set_state(U0 + eps * v)
+        _, fp = s.residual()
+        s.set_state(U0 - eps * v)
+        _, fm = s.residual()
+        s.set_state(U0)
+        gv = s.residual_vjp(w)
+        lhs, rhs = float(w @ ((fp - fm) / (2 * eps))), float(v @ gv)
+    dof = c.dof
+    value = dof * 4 * N / t_fwd / 1e9
+    out = {
+        "metric": "GDOF-stage-updates/s, mixed vs FP64 GPU at stated error; adjoint/forward cost",
+        "value": value,
+        "unit": "GDOF-stage/s",
+        "n_gpus": 1,
+        "steps": N,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_fwd / N,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random smooth velocity field)",
+        "config": {
+            "workload": c.name, "description": "32^3 periodic p=2, |k|<=3 unit-RMS velocity, "
+            "M=0.1, Re=1600, RK4 dt=1e-3", "nelem": c.mesh.nelem, "p": c.mesh.p, "dof": dof,
+            "scheme": "RK4", "parallelism": "single GPU",
+        },
+        "adjoint": {
+            "objective": "final kinetic energy (Gauss rule of the mass matrix)",
+            "J": J, "J_forward_run": J_fwd,
+            "forward_s": t_fwd,
+            "adjoint_s": t_adj,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "adjoint_state_only_s": t_adj_state,
+            "adjoint_state_only_over_forward": t_adj_state / t_fwd,
+            "checkpoint_interval": int(dgx.np.ceil(dgx.np.sqrt(N))),
+            "fd_directions": 20, "fd_eps": eps,
+            "fd_max_rel_err": max(errs), "fd_median_rel_err": float(dgx.np.median(errs)),
+            "dJdmu": dmu, "dJdmu_fd": fdmu, "dJdmu_rel_err": abs(dmu - fdmu) / abs(fdmu),
+            "residual_dot_test_rel_err": abs(lhs - rhs) / abs(rhs),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="config1",
-                    choices=["config0", "config1", "config2", "config3"])
+                    choices=["config0", "config1", "config2", "config3", "config4"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
@@ -200,6 +303,8 @@ def main():
     h = hip()
     if a.workload == "config3":
         return run_ensemble(a, h)
+    if a.workload == "config4":
+        return run_adjoint(a, h)
     case = {"config0": cases.config0, "config1": cases.config1, "config2": cases.config2}[a.workload]()
     s = dgx.Solver(case.mesh, case.params)
     s.set_state(case.U0)

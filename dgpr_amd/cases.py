@@ -124,3 +124,53 @@ def config3():
     """64 independent periodic cubes (sizes, orders, Mach and Re from seed 3),
     FP64 RK4, 50 steps each."""
     return ensemble_cases()
+
+
+def smooth_field_case(n=32, p=2, seed=4, M=0.1, Re=1600.0, kmax=3, dt=1e-3, nsteps=100,
+                      gamma=1.4, eta=6.0):
+    """Periodic cube [0, 2pi]^3 with a seeded random smooth velocity field:
+    Fourier modes 0 < |k| <= kmax with uniform random amplitudes and phases,
+    scaled to unit RMS over the nodes; rho = 1, p = 1 / (gamma M^2) (so
+    c = 1/M), mu = 1/Re.  State sampled at the equispaced Lagrange nodes of
+    the affine elements."""
+    nodes = dgx.box_nodes(1, n, n, n, 0.0, 0.0, 0.0, TWO_PI, TWO_PI, TWO_PI, 0.0, seed)
+    mesh = dgx.Mesh(p, 1, n, n, n, periodic_bounds(), nodes)
+    c = 1.0 / M
+    prm = dgx.make_reference_state(gamma=gamma, R=287.0, mu=1.0 / Re, c=c, rho=1.0, u=0.0,
+                                   v=0.0, w=0.0, eta=eta)
+    h = TWO_PI / n
+    xi = np.linspace(0.0, 1.0, p + 1)
+    ax = (np.arange(n)[:, None] + xi[None, :]) * h            # [elem index][node]
+    # node coordinates [k][j][i][bk][bj][bi] -> element-major [e][bf]
+    X = ax[None, None, :, None, None, :]
+    Y = ax[None, :, None, None, :, None]
+    Z = ax[:, None, None, :, None, None]
+    shape = (n, n, n, p + 1, p + 1, p + 1)
+    X, Y, Z = np.broadcast_to(X, shape), np.broadcast_to(Y, shape), np.broadcast_to(Z, shape)
+    rng = np.random.default_rng(seed)
+    vel = np.zeros((3,) + shape)
+    for kx in range(-kmax, kmax + 1):
+        for ky in range(-kmax, kmax + 1):
+            for kz in range(-kmax, kmax + 1):
+                if kx * kx + ky * ky + kz * kz == 0 or kx * kx + ky * ky + kz * kz > kmax * kmax:
+                    continue
+                amp = rng.uniform(-1.0, 1.0, 3)
+                ph = rng.uniform(0.0, TWO_PI, 3)
+                arg = kx * X + ky * Y + kz * Z
+                for d in range(3):
+                    vel[d] += amp[d] * np.cos(arg + ph[d])
+    vel /= np.sqrt(np.mean(np.sum(vel ** 2, axis=0)))
+    nel, nbf = n ** 3, (p + 1) ** 3
+    u = vel.reshape(3, nel, nbf)
+    P0 = prm.P
+    U = np.zeros((nel, 5, nbf))
+    U[:, 0] = 1.0
+    U[:, 1:4] = np.transpose(u, (1, 0, 2))
+    U[:, 4] = P0 / (gamma - 1.0) + 0.5 * np.sum(u ** 2, axis=0)
+    return Case("smooth_%d3_p%d" % (n, p), mesh, prm, U.reshape(-1), "RK4", dt, nsteps)
+
+
+def config4():
+    """32^3 periodic p=2, random smooth velocity (|k| <= 3, unit RMS, seed 4),
+    M = 0.1, RK4 dt = 1e-3, 100 steps: the adjoint workload."""
+    return smooth_field_case()
