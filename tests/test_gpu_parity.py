@@ -51,6 +51,15 @@ def bump_case(p, q, bounds, seed, amp=0.02, mu=1e-3, source=None):
     return cases.Case("bump_p%d_q%d" % (p, q), mesh, prm, U0, "tvdRK3", 1e-3, 5)
 
 
+def periodic_box_case(nxyz, p, seed):
+    """Periodic [0, 2pi]^3 box of nx x ny x nz elements with a perturbed
+    uniform state (Mach 0.2), RK4."""
+    base = cases.tgv_case(nxyz=nxyz, p=p, seed=seed, M=0.2)
+    U0 = perturbed_uniform(base.mesh, base.params, 0.05, seed)
+    return cases.Case("box_%dx%dx%d_p%d" % (tuple(nxyz) + (p,)), base.mesh, base.params, U0,
+                      "RK4", base.dt, 3)
+
+
 DEFAULT_BOUNDS = [dgx.SUBSONIC_INFLOW, dgx.SUBSONIC_OUTFLOW, dgx.INVISCID_WALL,
                   dgx.INVISCID_WALL, dgx.PERIODIC, dgx.PERIODIC]
 
@@ -72,6 +81,13 @@ CASES = {
     "bump_freestream_p2q1": lambda: bump_case(
         2, 1, [dgx.FREESTREAM, dgx.FREESTREAM, dgx.NO_SLIP_WALL, dgx.FREESTREAM,
                dgx.INVISCID_WALL, dgx.SUBSONIC_OUTFLOW], 23, source=[0.0, 0.3, -0.1, 0.05, 0.2]),
+    # degenerate periodic meshes: an element is its own neighbour across a
+    # periodic direction of one element (the face enumeration of
+    # geometry_shared.cpp:192-209 wraps onto the same element)
+    "single_element_p2": lambda: periodic_box_case((1, 1, 1), 2, 61),
+    "single_element_p3": lambda: periodic_box_case((1, 1, 1), 3, 62),
+    "ragged_1x2x1_p3": lambda: periodic_box_case((1, 2, 1), 3, 63),
+    "ragged_2x1x3_p1": lambda: periodic_box_case((2, 1, 3), 1, 64),
 }
 
 
