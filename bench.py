@@ -207,6 +207,12 @@ def run_adjoint(a, h):
         t0 = time.perf_counter()
         s.adjoint_rk4(c.dt, N, want_mu=False)
         t_adj_state = time.perf_counter() - t0
+        # every step-start state kept (no segment recomputation)
+        s.set_state(U0)
+        t0 = time.perf_counter()
+        _, g1, _ = s.adjoint_rk4(c.dt, N, interval=1, want_mu=False)
+        t_adj_all = time.perf_counter() - t0
+        same = float(dgx.np.max(dgx.np.abs(g1 - g)))
 
         def energy(U, mu=None):
             if mu is not None:
@@ -268,6 +274,8 @@ def run_adjoint(a, h):
             "adjoint_over_forward": t_adj / t_fwd,
             "adjoint_state_only_s": t_adj_state,
             "adjoint_state_only_over_forward": t_adj_state / t_fwd,
+            "adjoint_state_only_all_states_kept_over_forward": t_adj_all / t_fwd,
+            "all_states_kept_max_abs_grad_difference": same,
             "checkpoint_interval": int(dgx.np.ceil(dgx.np.sqrt(N))),
             "fd_directions": 20, "fd_eps": eps,
             "fd_max_rel_err": max(errs), "fd_median_rel_err": float(dgx.np.median(errs)),

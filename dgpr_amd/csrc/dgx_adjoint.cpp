@@ -261,7 +261,8 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
       for (int i = 0; i < 3; ++i)
         AHIP(c, stage(c, c->phys, S[i].p, T[i].p, S[0].p, S[i + 1].p, T[i + 1].p, kf[i].p,
                       cstage[i]));
-      AHIP(c, stage(c, c->phys, S[3].p, T[3].p, nullptr, nullptr, nullptr, kf[3].p, 0.0));
+      if (dJdmu)  // k4 only enters df/dmu
+        AHIP(c, stage(c, c->phys, S[3].p, T[3].p, nullptr, nullptr, nullptr, kf[3].p, 0.0));
       // reverse: l_(i+1) weights of k_(i+1), a_i adjoint through S_i
       AHIP(c, copy_state(c, asum.p, lam.p));
       for (int i = 3; i >= 0; --i)

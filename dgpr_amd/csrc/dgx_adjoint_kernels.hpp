@@ -14,41 +14,41 @@
 //   B: both adjoint records of each face point reversed through the trace
 //      production (trace interpolation, tangential derivatives, gradient
 //      traction), added to pvol.
-// Local Jacobians of the pointwise physics come from Dual<4> forward passes
-// (four input directions per pass) over the same templated functions that
-// the residual kernels use.
+// Local Jacobians of the pointwise physics: the parts that are linear in the
+// gradient arguments (stress from the velocity gradient, the tractions in the
+// face terms) are transposed in closed form; the state arguments go through
+// forward-mode dual passes (Dual<5> / Dual<4>, one direction per state
+// component) over the same templated functions the residual kernels use.
 
-// psi_x[k] = sum_o psi_out[o] d out[o] / d x[k] for the volume flux
-// H(Vq, gr) (17 inputs: Vq[5], gr[3][4]; 15 outputs)
+// Local adjoints of the pointwise physics.  The gradient arguments enter
+// linearly once the state is fixed (velocity gradient -> stress -> fluxes),
+// so their adjoints are written out; the state arguments go through dual
+// passes over the same templated functions the residual kernels run.
+
+// Volume flux H(Vq, gr) (dgx_physics.hpp): gx[0..4] = d/dVq, gx[5 + 4c + r]
+// = d/dgr[c][r] of sum psiH . H.
 __device__ __forceinline__ void volume_flux_vjp(const double* Vq, const double gr[3][4],
                                                 const double Ji[3][3], double gamma, double mu,
                                                 const double psiH[3][5], double* gx)
 {
-  using D4 = Dual<4>;
-  for (int chunk = 0; chunk < 5; ++chunk)
+  // state part: one pass with five directions, gr held fixed
   {
-    D4 vq[5], g[3][4], H[3][5];
+    using D5 = Dual<5>;
+    D5 vq[5], g[3][4], H[3][5];
 #pragma unroll
     for (int r = 0; r < 5; ++r)
     {
-      vq[r] = D4(Vq[r]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        vq[r].d[j] = (r == chunk * 4 + j) ? 1.0 : 0.0;
+      vq[r] = D5(Vq[r]);
+      vq[r].d[r] = 1.0;
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-      {
-        g[c][r] = D4(gr[c][r]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          g[c][r].d[j] = (5 + 4 * c + r == chunk * 4 + j) ? 1.0 : 0.0;
-      }
+        g[c][r] = D5(gr[c][r]);
     volume_flux(vq, g, Ji, gamma, mu, H);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 5; ++j)
     {
       double s = 0.0;
 #pragma unroll
@@ -56,52 +56,85 @@ __device__ __forceinline__ void volume_flux_vjp(const double* Vq, const double g
 #pragma unroll
         for (int r = 0; r < 5; ++r)
           s += psiH[d][r] * H[d][r].d[j];
-      if (chunk * 4 + j < 17)
-        gx[chunk * 4 + j] = s;
+      gx[j] = s;
     }
+  }
+  // gradient part: H[d][1+i] = tau[d][i] - ..., H[d][4] = u . tau[d] - ...,
+  // tau = mu (Lv + Lv^T) - 2/3 mu tr(Lv) I, Lv[i][d] = sum_c dv[c][i] Ji[c][d],
+  // dv[c][i] = (gr[c][1+i] - gr[c][0] u_i) / rho
+  const double ir = rcp(Vq[0]);
+  const double u[3] = {Vq[1] * ir, Vq[2] * ir, Vq[3] * ir};
+  double pt[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      pt[d][i] = psiH[d][1 + i] + psiH[d][4] * u[i];
+  const double ptr = -(2.0 / 3.0) * mu * (pt[0][0] + pt[1][1] + pt[2][2]);
+  double pL[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      pL[a][b] = mu * (pt[a][b] + pt[b][a]) + ((a == b) ? ptr : 0.0);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+  {
+    double s0 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+      const double pdv = pL[i][0] * Ji[c][0] + pL[i][1] * Ji[c][1] + pL[i][2] * Ji[c][2];
+      gx[5 + 4 * c + 1 + i] = pdv * ir;
+      s0 += pdv * u[i];
+    }
+    gx[5 + 4 * c] = -s0 * ir;
   }
 }
 
-// Interior face-point physics (16 inputs: UL[5], UR[5], vgL[3], vgR[3];
-// outputs G[5], S[6])
+// Directions per dual pass of the face-point physics: the dual Roe flux
+// keeps (1 + directions) doubles per live intermediate, so fewer directions
+// per pass trade passes for registers.
+#ifndef DGX_ADJ_FACE_DIRS
+#define DGX_ADJ_FACE_DIRS 2
+#endif
+
+// Interior face-point physics: gx[0..4] d/dUL, [5..9] d/dUR, [10..12] d/dvgL,
+// [13..15] d/dvgR of psiG . G + psiS . S.
 __device__ __forceinline__ void face_physics_vjp(const double* UL, const double* UR,
                                                  const double* vgL, const double* vgR,
                                                  const double* n, double lamL, double lamR,
                                                  const Phys& ph, const double* psiG,
                                                  const double* psiS, double* gx)
 {
-  using D4 = Dual<4>;
-  for (int chunk = 0; chunk < 4; ++chunk)
+  // traces (UL, UR: 10 directions), tractions fixed
+  constexpr int D = DGX_ADJ_FACE_DIRS;
+  using DD = Dual<D>;
+  for (int ch = 0; ch < (10 + D - 1) / D; ++ch)
   {
-    D4 ul[5], ur[5], gl[3], gR[3];
+    DD ul[5], ur[5], gl[3], gR[3];
 #pragma unroll
     for (int r = 0; r < 5; ++r)
     {
-      ul[r] = D4(UL[r]);
-      ur[r] = D4(UR[r]);
+      ul[r] = DD(UL[r]);
+      ur[r] = DD(UR[r]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < D; ++j)
       {
-        ul[r].d[j] = (r == chunk * 4 + j) ? 1.0 : 0.0;
-        ur[r].d[j] = (5 + r == chunk * 4 + j) ? 1.0 : 0.0;
+        ul[r].d[j] = (ch * D + j == r) ? 1.0 : 0.0;
+        ur[r].d[j] = (ch * D + j == 5 + r) ? 1.0 : 0.0;
       }
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-      gl[i] = D4(vgL[i]);
-      gR[i] = D4(vgR[i]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-      {
-        gl[i].d[j] = (10 + i == chunk * 4 + j) ? 1.0 : 0.0;
-        gR[i].d[j] = (13 + i == chunk * 4 + j) ? 1.0 : 0.0;
-      }
+      gl[i] = DD(vgL[i]);
+      gR[i] = DD(vgR[i]);
     }
-    FacePointT<D4> fo;
+    FacePointT<DD> fo;
     face_point_physics(true, ul, ur, gl, gR, n, lamL, lamR, ph, fo);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < D; ++j)
     {
       double s = 0.0;
 #pragma unroll
@@ -110,43 +143,67 @@ __device__ __forceinline__ void face_physics_vjp(const double* UL, const double*
 #pragma unroll
       for (int k = 0; k < 6; ++k)
         s += psiS[k] * fo.S[k].d[j];
-      gx[chunk * 4 + j] = s;
+      if (ch * D + j < 10)
+        gx[ch * D + j] = s;
     }
+  }
+  // tractions: G[1+i] -= h vg[i], G[4] -= h u . vg (h = 1/2 on interior faces)
+  const double irL = rcp(UL[0]), irR = rcp(UR[0]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+  {
+    gx[10 + i] = -0.5 * (psiG[1 + i] + psiG[4] * UL[1 + i] * irL);
+    gx[13 + i] = -0.5 * (psiG[1 + i] + psiG[4] * UR[1 + i] * irR);
   }
 }
 
-// Gradient traction vg = tau(grad u).n (16 inputs: gr[3][4], U[0..3];
-// 3 outputs)
+// Gradient traction vg = tau(grad u).n: gx[4c + r] = d/dgr[c][r],
+// gx[12 + r] = d/dU[r] (r < 4) of psivg . vg.
 __device__ __forceinline__ void traction_vjp(const double gr[3][4], const double* U,
                                              const double Ji[3][3], const double* n, double mu,
                                              const double* psivg, double* gx)
 {
-  using D4 = Dual<4>;
-  for (int chunk = 0; chunk < 4; ++chunk)
+  // state part: one pass with four directions (rho, m), gr fixed
   {
+    using D4 = Dual<4>;
     D4 g[3][4], u[5], vg[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-      {
         g[c][r] = D4(gr[c][r]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          g[c][r].d[j] = (4 * c + r == chunk * 4 + j) ? 1.0 : 0.0;
-      }
 #pragma unroll
     for (int r = 0; r < 5; ++r)
     {
       u[r] = D4(U[r]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        u[r].d[j] = (r < 4 && 12 + r == chunk * 4 + j) ? 1.0 : 0.0;
+      if (r < 4)
+        u[r].d[r] = 1.0;
     }
     grad_traction(g, u, Ji, n, mu, vg);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      gx[chunk * 4 + j] = psivg[0] * vg[0].d[j] + psivg[1] * vg[1].d[j] + psivg[2] * vg[2].d[j];
+      gx[12 + j] = psivg[0] * vg[0].d[j] + psivg[1] * vg[1].d[j] + psivg[2] * vg[2].d[j];
+  }
+  // gradient part: vg = mu (Ln + LTn) - 2/3 mu tr n with Ln[i] = sum_c dv_c[i] m_c,
+  // LTn[d] = sum_c Ji[c][d] (dv_c . n), tr = sum_c dv_c . Ji[c], m_c = Ji[c] . n
+  const double ir = rcp(U[0]);
+  const double u[3] = {U[1] * ir, U[2] * ir, U[3] * ir};
+  const double pn  = psivg[0] * n[0] + psivg[1] * n[1] + psivg[2] * n[2];
+  const double ptr = -(2.0 / 3.0) * mu * pn;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+  {
+    const double mc = Ji[c][0] * n[0] + Ji[c][1] * n[1] + Ji[c][2] * n[2];
+    const double kc = mu * (psivg[0] * Ji[c][0] + psivg[1] * Ji[c][1] + psivg[2] * Ji[c][2]);
+    double s0 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+      const double pdv = mu * psivg[i] * mc + kc * n[i] + ptr * Ji[c][i];
+      gx[4 * c + 1 + i] = pdv * ir;
+      s0 += pdv * u[i];
+    }
+    gx[4 * c] = -s0 * ir;
   }
 }
 
