@@ -54,12 +54,6 @@ enum
   DGX_TVDRK3 = 1
 };
 
-/* Arithmetic of the interpolation/projection contractions. */
-enum
-{
-  DGX_PREC_FP64 = 0 /* plain FP64 form */
-};
-
 /* parameters (solver_params.cpp:22-36), passed by value to the kernels
  * exactly as the reference passes it to cuda_residual. */
 typedef struct dgx_params
