@@ -3,7 +3,8 @@
     python -m dgpr_amd.build        (or python dgpr_amd/build.py)
 
 Compiles the HIP kernels and the C ABI with hipcc for gfx950 into
-``dgpr_amd/libdgx.so`` in-tree.  Packed-FP32 instructions are disabled
+``dgpr_amd/libdgx.so`` in-tree, and the ``dgpr_amd/dgx_drive`` program
+(dgx_drive.cpp, host C++) against it.  Packed-FP32 instructions are disabled
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``) for all device code.
 """
 import os
@@ -13,6 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdgx.so")
+DRIVER = os.path.join(HERE, "dgx_drive")
 BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["dgx_kernels.hip", "dgx_api.cpp", "dgx_ensemble.cpp", "dgx_adjoint.cpp",
@@ -58,7 +60,20 @@ def build(verbose: bool = False, jobs: int = 4, out: str = OUT, defines=()) -> s
     _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", out + ".tmp"] + objs,
          verbose)
     os.replace(out + ".tmp", out)
+    if out == OUT:
+        build_driver(verbose)
     return out
+
+
+def build_driver(verbose: bool = False) -> str:
+    """Build the ``dgx_drive`` program (host C++ only) next to libdgx.so,
+    linked to it with an $ORIGIN run path."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(CSRC, "dgx_drive.cpp"), "-o", DRIVER + ".tmp", "-L" + HERE, "-ldgx",
+           "-Wl,-rpath,$ORIGIN"]
+    _run(cmd, verbose)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -454,7 +454,8 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
           hc[c][i] = sc * (JiL[c][0] * Sm[0][i] + JiL[c][1] * Sm[1][i] + JiL[c][2] * Sm[2][i]);
       // adjoint of hc[c][0..3] from the normal / tangential parts
       double phc[3][4];
-      const int t1 = (nd == 0) ? 1 : 0, t2 = (nd == 2) ? 1 : 2;
+      // the remaining direction (not nd, not t1) takes ph2
+      const int t1 = (nd == 0) ? 1 : 0;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll

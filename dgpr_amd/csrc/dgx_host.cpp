@@ -198,6 +198,44 @@ dgx_status dgx_tgv_state(const dgx_mesh* m, double gamma, double rho0, double V0
   return DGX_OK;
 }
 
+// turbulent (solver_init.cpp:30-73): the free stream with, on each velocity
+// component i, amp * sum over wavelengths L = l/2, l/4, l/8, l/16 of
+// prod_j sin(2 pi (x_j - 0.3 i) / L), amp = |u_fr| / 10; density and total
+// energy stay at the free-stream values.
+dgx_status dgx_turbulent_state(const dgx_mesh* m, const dgx_params* prm, double l, double* U)
+{
+  if (!mesh_ok(m) || !prm || !U || !(l > 0.0) || !(prm->Ufr[0] > 0.0))
+    return DGX_ERR_INVALID;
+  int n = (int)m->p + 1, nbf = n * n * n;
+  uint64_t nelem = m->nx * m->ny * m->nz;
+  const double rho = prm->Ufr[0];
+  const double vfr[3] = {prm->Ufr[1] / rho, prm->Ufr[2] / rho, prm->Ufr[3] / rho};
+  const double amp = sqrt(vfr[0] * vfr[0] + vfr[1] * vfr[1] + vfr[2] * vfr[2]) / 10.0;
+  for (uint64_t e = 0; e < nelem; ++e)
+    for (int bf = 0; bf < nbf; ++bf)
+    {
+      double x[3];
+      node_position(m, e, bf, x);
+      double* Ue = U + e * 5 * (uint64_t)nbf;
+      Ue[0 * nbf + bf] = prm->Ufr[0];
+      Ue[4 * nbf + bf] = prm->Ufr[4];
+      for (int i = 0; i < 3; ++i)
+      {
+        double vel = vfr[i];
+        for (int k = 1; k <= 4; ++k)
+        {
+          const double L = l / (double)(1 << k);
+          double prod = 1.0;
+          for (int j = 0; j < 3; ++j)
+            prod *= sin(2.0 * kPi * ((x[j] - 0.3 * i) / L));
+          vel += amp * prod;
+        }
+        Ue[(1 + i) * nbf + bf] = vel * rho;
+      }
+    }
+  return DGX_OK;
+}
+
 dgx_status dgx_channel_state(const dgx_mesh* m, double gamma, double rho0, double Uc, double P0,
                              double frac, int kmax, double lx, double lz, uint64_t seed,
                              double* U)

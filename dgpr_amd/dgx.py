@@ -105,6 +105,7 @@ def lib():
         L.dgx_box_nodes.argtypes = [U64, U64, U64, U64, D, D, D, D, D, D, D, U64, PD]
         L.dgx_tgv_state.argtypes = [C.POINTER(_CMesh), D, D, D, D, D, U64, PD]
         L.dgx_channel_state.argtypes = [C.POINTER(_CMesh), D, D, D, D, D, C.c_int, D, D, U64, PD]
+        L.dgx_turbulent_state.argtypes = [C.POINTER(_CMesh), C.POINTER(Params), D, PD]
         L.dgx_write_dg.argtypes = [C.c_char_p, U64, C.POINTER(_CMesh), PD]
         L.dgx_read_dg_header.argtypes = [C.c_char_p, C.POINTER(U64), C.POINTER(_CMesh)]
         L.dgx_read_dg.argtypes = [C.c_char_p, PD, PD]
@@ -134,6 +135,7 @@ def lib():
                    "dgx_residual", "dgx_residual_device", "dgx_step", "dgx_step_async", "dgx_device_copy",
                    "dgx_sync", "dgx_get_stream", "dgx_get_sizes", "dgx_make_reference_state",
                    "dgx_bump_nodes", "dgx_box_nodes", "dgx_tgv_state", "dgx_channel_state",
+                   "dgx_turbulent_state",
                    "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg", "dgx_set_tuning",
                    "dgx_get_tuning", "dgx_autotune", "dgx_sample_output",
                    "dgx_sample_output_device", "dgx_update_mean", "dgx_get_mean",
@@ -235,6 +237,15 @@ def channel_state(mesh, gamma, rho0, Uc, P0, frac, kmax, lx, lz, seed):
     cm = mesh.cstruct()
     _chk(lib().dgx_channel_state(C.byref(cm), gamma, rho0, Uc, P0, frac, kmax, lx, lz, seed,
                                  _p(out)))
+    return out
+
+
+def turbulent_state(mesh, params, l):
+    """turbulent (solver_init.cpp:30-73): the free stream of ``params`` plus
+    four sine-product modes of wavelength l/2 .. l/16 on each velocity."""
+    out = np.zeros(mesh.state_size)
+    cm = mesh.cstruct()
+    _chk(lib().dgx_turbulent_state(C.byref(cm), C.byref(params), l, _p(out)))
     return out
 
 

@@ -310,6 +310,14 @@ dgx_status dgx_channel_state(const dgx_mesh* mesh, double gamma, double rho0,
                              double lx, double lz, uint64_t seed,
                              double* U_out);
 
+/* turbulent (solver_init.cpp:30-73), the "-turb" initial state of the
+ * driver: free-stream density and energy; velocity component i is the
+ * free-stream value plus amp * sum over L = l/2, l/4, l/8, l/16 of
+ * prod_j sin(2 pi (x_j - 0.3 i)/L), amp = |u_fr|/10, at the Lagrange nodes.
+ * U_out is a simstate [e][5][nbf]. */
+dgx_status dgx_turbulent_state(const dgx_mesh* mesh, const dgx_params* params,
+                               double l, double* U_out);
+
 /* .dg checkpoint file (io.cpp:33-159): write_state with one output named
  * "state", and read_state of that output.  dgx_read_dg_header fills
  * tstep and a mesh header (nodes NULL); dgx_read_dg then reads nodes and

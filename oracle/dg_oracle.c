@@ -1627,3 +1627,40 @@ void orc_update_mean(const double* state, double* mean, u64 n, u64 time_step, u6
   for (u64 i = 0; i < n; ++i)
     mean[i] = (mean[i] * old_range + state[i]) / new_range;
 }
+
+/* turbulent (solver_init.cpp:30-73): free-stream density and energy; each
+ * velocity component i gets amp * sin(2pi(x-0.3i)/L) sin(2pi(y-0.3i)/L)
+ * sin(2pi(z-0.3i)/L) for L = l/2, l/4, l/8, l/16, amp = |u_fr|/10, at the
+ * order-p Lagrange nodes. */
+void orc_turbulent(const orc_geom* g, const orc_params* prm, double l, double* U)
+{
+  const double twopi = 2. * ORC_PI, offst = 0.3;
+  const double rho = prm->Ufr[0];
+  const double u = prm->Ufr[1] / rho, v = prm->Ufr[2] / rho, w = prm->Ufr[3] / rho;
+  const double amp = sqrt(u * u + v * v + w * w) / 10.;
+  const double scale[4] = {0.5, 0.25, 0.125, 0.0625};
+  const int n = g->p + 1, nbf = n * n * n;
+  for (u64 e = 0; e < g->nelem; ++e)
+    for (int b = 0; b < nbf; ++b)
+    {
+      double r[3] = {orc_lagrange_node1d(b % n, g->p), orc_lagrange_node1d((b / n) % n, g->p),
+                     orc_lagrange_node1d(b / (n * n), g->p)};
+      double x[3];
+      ref2glo(g, e, r, x);
+      double* Ue = U + e * 5 * (u64)nbf;
+      Ue[b] = prm->Ufr[0];
+      Ue[4 * nbf + b] = prm->Ufr[4];
+      for (int i = 0; i < 3; ++i)
+      {
+        double vel = prm->Ufr[1 + i] / rho;
+        for (int k = 0; k < 4; ++k)
+        {
+          double L = scale[k] * l;
+          vel += amp * (sin(twopi * ((x[0] - i * offst) / L)) *
+                        sin(twopi * ((x[1] - i * offst) / L)) *
+                        sin(twopi * ((x[2] - i * offst) / L)));
+        }
+        Ue[(1 + i) * nbf + b] = vel * rho;
+      }
+    }
+}

@@ -77,6 +77,7 @@ def lib():
         L.orc_set_plant.argtypes = [C.c_int]
         L.orc_eval_output.argtypes = [C.c_int, U64, PD, C.c_int, C.c_int, D, PD]
         L.orc_update_mean.argtypes = [PD, PD, U64, U64, U64]
+        L.orc_turbulent.argtypes = [C.c_void_p, C.POINTER(Params), D, PD]
         L.RHS = C.CFUNCTYPE(None, PD, PD, C.c_void_p)
         L.orc_step_generic.argtypes = [C.c_int, D, U64, PD, L.RHS, C.c_void_p]
     return _lib
@@ -299,6 +300,12 @@ class Geometry:
         if st:
             raise RuntimeError("oracle residual failed (status %d)" % st)
         return R, f
+
+    def turbulent(self, prm, l):
+        """turbulent (solver_init.cpp:30-73) on this mesh; returns U."""
+        U = np.zeros(self.state_size())
+        lib().orc_turbulent(self.h, C.byref(prm), l, _p(U))
+        return U
 
     def step(self, prm, U, dt, scheme="RK4", nsteps=1):
         """Advance a copy of U by nsteps; returns (U, per-step RMS(R) norms)."""
