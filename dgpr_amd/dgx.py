@@ -317,6 +317,22 @@ class Solver:
         _chk(lib().dgx_get_state(self.ctx, _p(U)), self.ctx)
         return U
 
+    def set_state_device(self, ptr):
+        """Load the state from a device buffer (simstate layout, nstate
+        doubles) given as an integer address; returns after completion."""
+        _chk(lib().dgx_set_state_device(self.ctx, C.c_void_p(int(ptr))), self.ctx)
+
+    def get_state_device(self, ptr):
+        """Store the state into a device buffer given as an integer address."""
+        _chk(lib().dgx_get_state_device(self.ctx, C.c_void_p(int(ptr))), self.ctx)
+
+    def residual_device(self, U_ptr, R_ptr, f_ptr):
+        """dgx_residual_device on device buffers given as integer addresses
+        (R_ptr may be 0); returns after completion."""
+        _chk(lib().dgx_residual_device(self.ctx, C.c_void_p(int(U_ptr)),
+                                       C.c_void_p(int(R_ptr)) if R_ptr else None,
+                                       C.c_void_p(int(f_ptr))), self.ctx)
+
     def residual(self):
         """(R, f) of the current state in simstate layout (solve.cpp:150-807)."""
         R = np.zeros(self.nstate)
