@@ -12,6 +12,9 @@ Prints one JSON line:
   value      = DOF * stages * steps / time / 1e9            [GDOF-stage/s]
   roofline   = algorithmic HBM bytes per stage-kernel launch / average launch
                time (HIP events on the library's stream) vs 8 TB/s
+  mixed      = the same workload in the mixed-precision form (FP32 face trace
+               records): throughput, speedup over FP64, per-field L2/Linf
+               errors and kinetic-energy dissipation rate against the FP64 run
   cpu_baseline = the oracle (CPU restatement of the reference's dense
                residual + RK) on a bounded sample, timed on this host's cores
 
@@ -286,6 +289,78 @@ def run_adjoint(a, h):
     print(json.dumps(out), flush=True)
 
 
+def kinetic_rate(s, case):
+    """Kinetic energy of the current state and the dissipation rate -dK/dt
+    over one more step (central in the step: (K_n - K_n+1) / dt)."""
+    k0 = s.kinetic_energy()
+    s.step(case.dt, case.scheme, 1, norms=False)
+    k1 = s.kinetic_energy()
+    return k0, (k0 - k1) / case.dt
+
+
+def run_mixed(a, h, s, case, U64, value64, stream, ev0, ev1):
+    """The mixed-precision form (FP32 face trace records) on the same
+    workload: the same untimed warmup, tuning and step, the same timed
+    steps, then per-field L2/Linf errors against the FP64 run (both from the
+    same initial state after the same number of steps) and the kinetic-
+    energy dissipation rate of both forms over one more step."""
+    np = dgx.np
+    total = a.warmup + 1 + a.steps
+    # FP64 reference quantities at the end of the timed run
+    k64, eps64 = kinetic_rate(s, case)
+    s.set_state(case.U0)
+    s.precision = "mixed"
+    s.step_async(case.dt, case.scheme, a.warmup)
+    s.sync()
+    if a.autotune:
+        waves, _ = s.autotune(reps=3)
+    else:
+        waves = s.stage_waves
+    s.step_async(case.dt, case.scheme, 1)
+    s.sync()
+    h.hipDeviceSynchronize()
+    h.hipEventRecord(ev0, stream)
+    s.step_async(case.dt, case.scheme, a.steps)
+    h.hipEventRecord(ev1, stream)
+    h.hipEventSynchronize(ev1)
+    s.sync()
+    ms = C.c_float()
+    h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    gpu_s = ms.value * 1e-3
+    value = case.dof * case.stages * a.steps / gpu_s / 1e9
+    Um = s.get_state()
+    kmx, epsmx = kinetic_rate(s, case)
+    s.precision = "fp64"
+    A = Um.reshape(case.mesh.nelem, 5, -1)
+    B = U64.reshape(case.mesh.nelem, 5, -1)
+    fields = {}
+    for r, nm in enumerate(["rho", "rho_u", "rho_v", "rho_w", "rho_E"]):
+        d = A[:, r] - B[:, r]
+        fields[nm] = {
+            "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(B[:, r])),
+            "linf_rel": float(np.abs(d).max() / np.abs(B[:, r]).max()),
+            "l2_rel_to_state": float(np.linalg.norm(d) / np.linalg.norm(U64)),
+        }
+    return {
+        "value": value,
+        "unit": "GDOF-stage/s",
+        "speedup_vs_fp64": value / value64,
+        "ms_per_step": 1e3 * gpu_s / a.steps,
+        "avg_launch_us": gpu_s / (a.steps * case.stages) * 1e6,
+        "stage_waves_per_simd": waves,
+        "form": "FP32 face trace records; geometry, fluxes, Roe, inverse mass, RK in FP64",
+        "error_vs_fp64": {
+            "steps": total,
+            "state_l2_rel": float(np.linalg.norm(Um - U64) / np.linalg.norm(U64)),
+            "fields": fields,
+            "kinetic_energy_rel": abs(kmx - k64) / abs(k64),
+            "ke_dissipation_rate_fp64": eps64,
+            "ke_dissipation_rate_mixed": epsmx,
+            "ke_dissipation_rate_rel": abs(epsmx - eps64) / abs(eps64),
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,6 +372,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default occupancy target instead of timing all of them")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision run of the same workload")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one copy kernel of this size (traffic calibration)")
     a = ap.parse_args()
@@ -363,6 +440,7 @@ def main():
 
     U = s.get_state()
     finite = bool(dgx.np.isfinite(U).all())
+    mixed = None if a.no_mixed else run_mixed(a, h, s, case, U, value, stream, ev0, ev1)
     s.close()
 
     # HBM bytes and executed FP64 flops per stage-kernel launch, from the
@@ -426,6 +504,8 @@ def main():
             "flops_per_dof_stage": flops / case.dof,
         },
     }
+    if mixed is not None:
+        out["mixed"] = mixed
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)

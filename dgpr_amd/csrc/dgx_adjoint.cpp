@@ -110,6 +110,7 @@ hipError_t stage(dgx_ctx* c, const Phys& ph, const double* Vin, const double* Ti
   sa.cb = 0.0;
   sa.cf = cf;
   sa.accmode = 0;
+  sa.trace32 = 0;
   return rhs_dispatch(c->p, c->q, mesh_args(c), ph, sa, c->lam, c->status, c->stage_waves,
                       c->stream);
 }
@@ -160,6 +161,8 @@ dgx_status dgx_residual_vjp(dgx_ctx* c, const double* w_host, double* g_host)
     return DGX_ERR_INVALID;
   if (!all_periodic(c))
     return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs a periodic mesh");
+  if (c->prec != DGX_PREC_FP64)
+    return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs DGX_PREC_FP64");
   AdjScratch s;
   DevBuf a, b;
   AHIP(c, alloc_scratch(c, s));
@@ -186,6 +189,8 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
     return DGX_ERR_INVALID;
   if (!all_periodic(c))
     return afail(c, DGX_ERR_UNSUPPORTED, "the discrete adjoint needs a periodic mesh");
+  if (c->prec != DGX_PREC_FP64)
+    return afail(c, DGX_ERR_UNSUPPORTED, "the discrete adjoint needs DGX_PREC_FP64");
   if (dJdmu && !(c->phys.mu > 0.0))
     return afail(c, DGX_ERR_INVALID, "dJ/dmu needs mu > 0");
   const uint64_t K = interval ? interval : (uint64_t)std::ceil(std::sqrt((double)nsteps));

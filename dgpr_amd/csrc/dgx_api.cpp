@@ -225,6 +225,7 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
       return bail("hipMalloc(traces)", e);
   c->tu_valid = false;
   c->stage_waves = default_stage_waves(c->p, c->q);
+  c->prec = DGX_PREC_FP64;
   c->bytes = 5 * sb + 3 * tb + (nn + nlam + (size_t)c->npartial) * sizeof(double) + 16;
   if ((e = hipMemcpyAsync(c->nodes, mesh->nodes, nn * sizeof(double), hipMemcpyHostToDevice,
                           c->stream)) != hipSuccess)
@@ -359,6 +360,7 @@ static hipError_t rhs(dgx_ctx* c, const double* Vin, const double* Tin, const do
   sa.cb = cb;
   sa.cf = cf;
   sa.accmode = accmode;
+  sa.trace32 = c->prec == DGX_PREC_MIXED;
   return rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status, c->stage_waves,
                       c->stream);
 }
@@ -370,8 +372,8 @@ hipError_t dgx::ensure_traces(dgx_ctx* c)
 {
   if (c->tu_valid)
     return hipSuccess;
-  hipError_t e =
-      trace_dispatch(c->p, c->q, mesh_args(c), c->phys, c->U, c->TU, c->status, c->stream);
+  hipError_t e = trace_dispatch(c->p, c->q, mesh_args(c), c->phys, c->U, c->TU, c->status,
+                                c->stream, c->prec == DGX_PREC_MIXED);
   if (e == hipSuccess)
     c->tu_valid = true;
   return e;
@@ -398,7 +400,7 @@ dgx_status dgx_residual_device(dgx_ctx* c, const double* U_dev, double* R_dev, d
     return DGX_ERR_INVALID;
   HIPCHK(c, basis_dispatch(c->p, U_dev, c->Ua, c->nelem, 0, c->stream));
   HIPCHK(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys, c->Ua, c->TUa, c->status,
-                           c->stream));
+                           c->stream, c->prec == DGX_PREC_MIXED));
   HIPCHK(c, rhs(c, c->Ua, c->TUa, nullptr, nullptr, nullptr, nullptr, c->fs, 0, 0, 0, 0));
   HIPCHK(c, resid_out_dispatch(c->p, c->q, mesh_args(c), c->fs, f_dev, R_dev, nullptr,
                                c->stream));
@@ -428,9 +430,10 @@ dgx_status dgx_residual(dgx_ctx* c, double* R_host, double* f_host)
 // instead of recomputing.
 int dgx::stage_plan(const dgx_ctx* c, int scheme, double dt, StageArgs sa[4])
 {
-  auto set = [](StageArgs& a, const double* Vin, const double* Tin, const double* Vbase,
-                double* acc, double* Vout, double* Tout, double ca, double cb, double cf,
-                int accmode) {
+  const int t32 = c->prec == DGX_PREC_MIXED;
+  auto set = [t32](StageArgs& a, const double* Vin, const double* Tin, const double* Vbase,
+                   double* acc, double* Vout, double* Tout, double ca, double cb, double cf,
+                   int accmode) {
     a.Vin = Vin;
     a.Tin = Tin;
     a.Vbase = Vbase;
@@ -442,6 +445,7 @@ int dgx::stage_plan(const dgx_ctx* c, int scheme, double dt, StageArgs sa[4])
     a.cb = cb;
     a.cf = cf;
     a.accmode = accmode;
+    a.trace32 = t32;
   };
   if (scheme == DGX_RK4)
   {
@@ -533,6 +537,26 @@ dgx_status dgx_step(dgx_ctx* c, int scheme, double dt, uint64_t nsteps, double* 
 }
 
 }  // extern "C"
+
+dgx_status dgx_set_precision(dgx_ctx* c, int precision)
+{
+  if (!c || (precision != DGX_PREC_FP64 && precision != DGX_PREC_MIXED))
+    return DGX_ERR_INVALID;
+  if (precision != c->prec)
+  {
+    c->prec = precision;
+    c->tu_valid = false;
+  }
+  return DGX_OK;
+}
+
+dgx_status dgx_get_precision(const dgx_ctx* c, int* precision)
+{
+  if (!c || !precision)
+    return DGX_ERR_INVALID;
+  *precision = c->prec;
+  return DGX_OK;
+}
 
 dgx_status dgx_set_tuning(dgx_ctx* c, int stage_waves)
 {

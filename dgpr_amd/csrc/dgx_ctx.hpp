@@ -33,6 +33,7 @@ struct dgx_ctx
   double* TUb;
   bool tu_valid;    // TU holds the traces of the current U
   int stage_waves;  // occupancy target of the stage kernel (1..3)
+  int prec;         // DGX_PREC_FP64 or DGX_PREC_MIXED (float trace records)
   double* mean;     // running mean (Gauss-nodal), allocated on first use
   int* status;      // kernel-side error flags
   int npartial;

@@ -267,6 +267,9 @@ dgx_status dgx_ensemble_step(dgx_ensemble* en, int scheme, const double* dt, uin
 {
   if (!en || !dt || (scheme != DGX_RK4 && scheme != DGX_TVDRK3))
     return DGX_ERR_INVALID;
+  for (const dgx_ctx* c : en->ctx)
+    if (c->prec != DGX_PREC_FP64)
+      return efail(en, DGX_ERR_UNSUPPORTED, "ensemble steps need DGX_PREC_FP64 members");
   dgx_status st = plan(en, scheme, dt);
   if (st != DGX_OK)
     return st;
@@ -338,6 +341,9 @@ dgx_status dgx_ensemble_autotune(dgx_ensemble* en, int scheme, const double* dt,
 {
   if (!en || !dt || reps < 1 || (scheme != DGX_RK4 && scheme != DGX_TVDRK3))
     return DGX_ERR_INVALID;
+  for (const dgx_ctx* c : en->ctx)
+    if (c->prec != DGX_PREC_FP64)
+      return efail(en, DGX_ERR_UNSUPPORTED, "ensemble steps need DGX_PREC_FP64 members");
   dgx_status st = plan(en, scheme, dt);
   if (st != DGX_OK)
     return st;

@@ -292,20 +292,22 @@ struct FaceDims
 // the point of face f of e and of face f^1 of its neighbour coincide
 // (basis.cpp:180-192).  On a domain-boundary face the boundary state Ub
 // (bound_states.cpp:24-130) of the trace is stored as well, after the trace
-// records of all elements: Tb[boundary face][5][(p+1)^2].  Called by every
+// records of all elements: Tb[boundary face][5][(p+1)^2].  TT is the stored
+// type: double, or float for the mixed-precision form (the records are
+// computed in double and rounded once on the store).  Called by every
 // thread of the block; sR needs 4*64 doubles.
-template <int P, int Q>
+template <int P, int Q, typename TT>
 __device__ __forceinline__ void produce_traces(const MeshArgs& mesh, const Phys& ph,
                                                const PTab& pt, const QTab& qt,
                                                const double (*sV)[5][FaceDims<P>::NB],
                                                const double (*sX)[(Q + 1) * (Q + 1) * (Q + 1) * 3],
                                                const int (*sNb)[6], const int (*sBt)[6],
-                                               double* sR, int e0, double* __restrict__ Tout,
+                                               double* sR, int e0, TT* __restrict__ Tout,
                                                int* status)
 {
   using FD = FaceDims<P>;
   constexpr int N = FD::N, NF = FD::NF;
-  double* Tb = Tout + (size_t)mesh.nelem * 6 * 8 * NF;
+  TT* Tb = Tout + (size_t)mesh.nelem * 6 * 8 * NF;
   const int tid = threadIdx.x;
   bool ok = true;
   for (int pass = 0; pass < FD::NPASS; ++pass)
@@ -372,22 +374,22 @@ __device__ __forceinline__ void produce_traces(const MeshArgs& mesh, const Phys&
       }
       double vg[3];
       grad_traction(gr, U, Ji, nv, ph.mu, vg);
-      double* T = Tout + ((size_t)ee * 6 + f) * 8 * NF + fpt;
+      TT* T = Tout + ((size_t)ee * 6 + f) * 8 * NF + fpt;
 #pragma unroll
       for (int r = 0; r < 5; ++r)
-        T[r * NF] = U[r];
+        T[r * NF] = (TT)U[r];
 #pragma unroll
       for (int i = 0; i < 3; ++i)
-        T[(5 + i) * NF] = vg[i];
+        T[(5 + i) * NF] = (TT)vg[i];
       const int nb = sNb[l][f];
       if (nb < 0)
       {
         double Ub[5];
         ok = boundary_state(sBt[l][f], U, nv, ph, Ub) && ok;
-        double* B = Tb + (size_t)(-1 - nb) * 5 * NF + fpt;
+        TT* B = Tb + (size_t)(-1 - nb) * 5 * NF + fpt;
 #pragma unroll
         for (int r = 0; r < 5; ++r)
-          B[r * NF] = Ub[r];
+          B[r * NF] = (TT)Ub[r];
       }
     }
     __syncthreads();
@@ -398,10 +400,10 @@ __device__ __forceinline__ void produce_traces(const MeshArgs& mesh, const Phys&
 
 // Trace records of a Gauss-nodal state V (the first stage after the state
 // was set from outside, and stand-alone residual evaluations).
-template <int P, int Q>
+template <int P, int Q, typename TT>
 __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
                                                        const double* __restrict__ V,
-                                                       double* __restrict__ Tout, int* status)
+                                                       TT* __restrict__ Tout, int* status)
 {
   using FD = FaceDims<P>;
   constexpr int NB = FD::NB, EPW = FD::EPW;
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
         (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
   }
   __syncthreads();
-  produce_traces<P, Q>(mesh, ph, sPT, sQT, sV, sX, sNb, sBt, sR, e0, Tout, status);
+  produce_traces<P, Q, TT>(mesh, ph, sPT, sQT, sV, sX, sNb, sBt, sR, e0, Tout, status);
 }
 
 // One full right-hand-side evaluation f_g = B3 f of EPW elements fused with
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
 // holds the BR2 lifting coefficient of element e's side of its face f.
 // Body of the stage kernel for block `block` of one mesh: shared by the
 // single-mesh kernel and the batched ensemble kernel.
-template <int P, int Q>
+template <int P, int Q, typename TT>
 __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
                                            const StageArgs& sa, const double* __restrict__ lam,
                                            int* status, int block)
@@ -464,6 +466,7 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
   const int nd0 = tid % NB;
   const int e0  = block * EPW;
   const int e   = e0 + le;
+  const TT* __restrict__ Tin = reinterpret_cast<const TT*>(sa.Tin);
   const bool vact = (le < EPW) && (e < mesh.nelem);
   face_table<EPW>(mesh, e0, sNb, sBt);
 
@@ -572,7 +575,7 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
     if (fact)
     {
       const int en = sNb[l][f];
-      const double* To = sa.Tin + ((size_t)ee * 6 + f) * 8 * NF + fpt;
+      const TT* To = Tin + ((size_t)ee * 6 + f) * 8 * NF + fpt;
       double UL[5], vgL[3];
 #pragma unroll
       for (int r = 0; r < 5; ++r)
@@ -598,7 +601,7 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
       double lamR = 0.0;
       if (en >= 0)
       {
-        const double* Tn = sa.Tin + ((size_t)en * 6 + (f ^ 1)) * 8 * NF + fpt;
+        const TT* Tn = Tin + ((size_t)en * 6 + (f ^ 1)) * 8 * NF + fpt;
 #pragma unroll
         for (int r = 0; r < 5; ++r)
           Ub[r] = Tn[r * NF];
@@ -609,7 +612,7 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
       }
       else
       {
-        const double* B = sa.Tin + (size_t)mesh.nelem * 6 * 8 * NF + (size_t)(-1 - en) * 5 * NF + fpt;
+        const TT* B = Tin + (size_t)mesh.nelem * 6 * 8 * NF + (size_t)(-1 - en) * 5 * NF + fpt;
 #pragma unroll
         for (int r = 0; r < 5; ++r)
           Ub[r] = B[r * NF];
@@ -733,18 +736,20 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
   if (sa.Tout)
   {
     __syncthreads();
-    produce_traces<P, Q>(mesh, ph, pt, qt, sV, sX, sNb, sBt, sR, e0, sa.Tout, status);
+    produce_traces<P, Q, TT>(mesh, ph, pt, qt, sV, sX, sNb, sBt, sR, e0,
+                             reinterpret_cast<TT*>(sa.Tout), status);
   }
 }
 
 // W is the occupancy target (waves per SIMD) handed to the register
-// allocator; the launch picks one of the compiled targets per (P, Q).
-template <int P, int Q, int W>
+// allocator; the launch picks one of the compiled targets per (P, Q).  TT is
+// the type of the trace records read and written (float: mixed precision).
+template <int P, int Q, int W, typename TT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
                           int* status)
 {
-  stage_body<P, Q>(mesh, ph, sa, lam, status, blockIdx.x);
+  stage_body<P, Q, TT>(mesh, ph, sa, lam, status, blockIdx.x);
 }
 
 // Batched form: the blocks of several independent meshes of the same (P, Q)
@@ -765,7 +770,7 @@ void dgx_rhs_batch_kernel(const BatchItem* __restrict__ items, int nitems, int* 
       hi = mid - 1;
   }
   const BatchItem& it = items[lo];
-  stage_body<P, Q>(it.mesh, it.ph, it.sa, it.lam, status, b - it.block0);
+  stage_body<P, Q, double>(it.mesh, it.ph, it.sa, it.lam, status, b - it.block0);
 }
 
 // BR2 lifting coefficient of each element's side of each of its faces:
@@ -1163,24 +1168,33 @@ __global__ __launch_bounds__(256) void dgx_mean_kernel(const double* __restrict_
 
 // ---- launch helpers (host) -----------------------------------------------
 
-template <int P, int Q>
-hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
-                      int* status, int waves, hipStream_t st)
+template <int P, int Q, typename TT>
+hipError_t launch_rhs_t(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
+                        int* status, int waves, hipStream_t st)
 {
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
   if (waves == 1)
-    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 1>), dim3(nblk), dim3(64), 0, st, m, ph, sa,
-                       lam, status);
+    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 1, TT>), dim3(nblk), dim3(64), 0, st, m, ph,
+                       sa, lam, status);
   else if (waves == 2)
-    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 2>), dim3(nblk), dim3(64), 0, st, m, ph, sa,
-                       lam, status);
+    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 2, TT>), dim3(nblk), dim3(64), 0, st, m, ph,
+                       sa, lam, status);
   else if (waves == 3)
-    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 3>), dim3(nblk), dim3(64), 0, st, m, ph, sa,
-                       lam, status);
+    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 3, TT>), dim3(nblk), dim3(64), 0, st, m, ph,
+                       sa, lam, status);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
+}
+
+template <int P, int Q>
+hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
+                      int* status, int waves, hipStream_t st)
+{
+  if (sa.trace32)
+    return launch_rhs_t<P, Q, float>(m, ph, sa, lam, status, waves, st);
+  return launch_rhs_t<P, Q, double>(m, ph, sa, lam, status, waves, st);
 }
 
 template <int P, int Q>
@@ -1203,12 +1217,16 @@ hipError_t launch_rhs_batch(const BatchItem* items, int nitems, int nblk, int* s
 
 template <int P, int Q>
 hipError_t launch_trace(const MeshArgs& m, const Phys& ph, const double* V, double* Tout,
-                        int* status, hipStream_t st)
+                        int* status, hipStream_t st, bool trace32)
 {
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
-  hipLaunchKernelGGL((dgx_trace_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, V, Tout,
-                     status);
+  if (trace32)
+    hipLaunchKernelGGL((dgx_trace_kernel<P, Q, float>), dim3(nblk), dim3(64), 0, st, m, ph, V,
+                       reinterpret_cast<float*>(Tout), status);
+  else
+    hipLaunchKernelGGL((dgx_trace_kernel<P, Q, double>), dim3(nblk), dim3(64), 0, st, m, ph, V,
+                       Tout, status);
   return hipGetLastError();
 }
 
@@ -1292,9 +1310,9 @@ int stage_blocks(int p, int nelem)
 }
 
 hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
-                          double* Tout, int* status, hipStream_t st)
+                          double* Tout, int* status, hipStream_t st, bool trace32)
 {
-#define CALL_TR(P, Q) launch_trace<P, Q>(m, ph, V, Tout, status, st)
+#define CALL_TR(P, Q) launch_trace<P, Q>(m, ph, V, Tout, status, st, trace32)
   DGX_DISPATCH_PQ(p, q, CALL_TR)
 #undef CALL_TR
 }

@@ -60,6 +60,8 @@ struct MeshArgs
 // face trace records of Vin; Tout (optional, only with Vout) receives those
 // of Vout.  A trace record set is [e][6 faces][8][(p+1)^2]: the trace (5)
 // and the gradient traction tau(grad u).n with the own outward normal (3).
+// With trace32 set the records are floats (the mixed-precision form; the
+// buffers keep their double-sized allocation and use its first half).
 struct StageArgs
 {
   const double* Vin;
@@ -71,6 +73,7 @@ struct StageArgs
   double* Tout;
   double ca, cb, cf;
   int accmode;
+  int trace32;
 };
 
 // One member of a batched stage launch: its mesh, physics, stage arguments
@@ -97,7 +100,7 @@ int stage_blocks(int p, int nelem);
 // face trace records Tout of the Gauss-nodal state V (and the boundary
 // states of its domain-boundary faces, see trace_doubles)
 hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
-                          double* Tout, int* status, hipStream_t st);
+                          double* Tout, int* status, hipStream_t st, bool trace32 = false);
 // doubles of one trace record set: nelem*6*8*(p+1)^2 trace records followed
 // by 5*(p+1)^2 boundary-state values for each of the mesh's boundary faces
 size_t trace_doubles(int p, int nx, int ny, int nz);

@@ -40,6 +40,8 @@ BTYPES = {"periodic": PERIODIC, "freestream": FREESTREAM, "inviscid_wall": INVIS
 # time_scheme (time.cpp:23-27)
 RK4, TVDRK3 = 0, 1
 SCHEMES = {"RK4": RK4, "tvdRK3": TVDRK3}
+PREC_FP64, PREC_MIXED = 0, 1
+PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "HIP error",
           4: "NaN residual norm", 5: "I/O error", 6: "no HIP device"}
@@ -129,6 +131,8 @@ def lib():
         L.dgx_update_mean.argtypes = [P_ctx, U64, U64]
         L.dgx_get_mean.argtypes = [P_ctx, PD]
         L.dgx_get_tuning.argtypes = [P_ctx, C.POINTER(C.c_int)]
+        L.dgx_set_precision.argtypes = [P_ctx, C.c_int]
+        L.dgx_get_precision.argtypes = [P_ctx, C.POINTER(C.c_int)]
         L.dgx_autotune.argtypes = [P_ctx, C.c_int, C.POINTER(C.c_int), PD]
         for nm in ("dgx_create", "dgx_destroy", "dgx_set_params", "dgx_set_state",
                    "dgx_get_state", "dgx_set_state_device", "dgx_get_state_device",
@@ -137,7 +141,8 @@ def lib():
                    "dgx_bump_nodes", "dgx_box_nodes", "dgx_tgv_state", "dgx_channel_state",
                    "dgx_turbulent_state",
                    "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg", "dgx_set_tuning",
-                   "dgx_get_tuning", "dgx_autotune", "dgx_sample_output",
+                   "dgx_get_tuning", "dgx_autotune", "dgx_set_precision", "dgx_get_precision",
+                   "dgx_sample_output",
                    "dgx_sample_output_device", "dgx_update_mean", "dgx_get_mean",
                    "dgx_ensemble_create", "dgx_ensemble_destroy", "dgx_ensemble_member",
                    "dgx_ensemble_step", "dgx_ensemble_sync", "dgx_ensemble_get_stream",
@@ -412,6 +417,17 @@ class Solver:
         M = np.zeros(self.nstate)
         _chk(lib().dgx_get_mean(self.ctx, _p(M)), self.ctx)
         return M
+
+    @property
+    def precision(self):
+        """'fp64' or 'mixed' (FP32 face trace records, the rest FP64)."""
+        v = C.c_int()
+        _chk(lib().dgx_get_precision(self.ctx, C.byref(v)), self.ctx)
+        return {b: a for a, b in PRECISIONS.items()}[v.value]
+
+    @precision.setter
+    def precision(self, name):
+        _chk(lib().dgx_set_precision(self.ctx, PRECISIONS[name]), self.ctx)
 
     def autotune(self, reps=5):
         """Times each compiled occupancy target on this mesh, keeps the

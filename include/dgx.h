@@ -246,6 +246,25 @@ dgx_status dgx_residual_vjp(dgx_ctx* ctx, const double* w_host, double* g_host);
 dgx_status dgx_adjoint_rk4(dgx_ctx* ctx, double dt, uint64_t nsteps, uint64_t interval,
                            double* J, double* dJdU0_host, double* dJdmu);
 
+/* ---- precision form (no counterpart in the reference: its CPU solver is
+ * FP64 and its CUDA driver FP32 throughout, cudg.cu:26-27) ----------------- */
+
+typedef enum dgx_precision
+{
+  DGX_PREC_FP64  = 0, /* everything in FP64 (default)                      */
+  DGX_PREC_MIXED = 1  /* FP32 face trace records, the rest in FP64         */
+} dgx_precision;
+
+/* DGX_PREC_MIXED stores the face trace records that cross element faces
+ * between stages (the interpolated trace state and the gradient traction,
+ * computed in FP64 and rounded once) in FP32; geometry, the Roe and viscous
+ * fluxes, pressure/energy recovery, the inverse mass and the RK accumulation
+ * stay FP64.  Changing the form drops the stored records (the next stage
+ * recomputes them).  The adjoint entry points and ensemble steps need
+ * DGX_PREC_FP64 (DGX_ERR_UNSUPPORTED otherwise). */
+dgx_status dgx_set_precision(dgx_ctx* ctx, int precision);
+dgx_status dgx_get_precision(const dgx_ctx* ctx, int* precision);
+
 /* ---- launch tuning (no counterpart in the reference, whose kernels have
  * fixed launch shapes, solve_cuda.cu:1235-1282) ---------------------------- */
 

@@ -357,6 +357,8 @@ class device_geometry
 
   dgx_ctx* ctx() const { return ctx_; }
   void set_params(const parameters& prm) { check(dgx_set_params(ctx_, &prm), ctx_); }
+  // DGX_PREC_FP64 (default) or DGX_PREC_MIXED (FP32 face trace records)
+  void set_precision(int prec) { check(dgx_set_precision(ctx_, prec), ctx_); }
   // shuffle_state + H2D (cudg.cu:97-106)
   void upload(const simstate& U) { check(dgx_set_state(ctx_, U.U.data()), ctx_); }
   // D2H + unshuffle_state (cudg.cu:131-136)
