@@ -15,11 +15,14 @@
 //   l1 = dt/6 l + dt/2 a1,  a0 = f_V(V)^T l1
 //   adjoint of V = l + a0 + a1 + a2 + a3,
 // and dJ/dmu gains sum_i l_i . df/dmu(S_(i-1)), df/dmu = (f(S; mu) - f(S; 0))/mu
-// because every viscous and BR2 term is linear in mu.  Checkpoints every
-// `interval` steps (default ceil(sqrt(N))); each segment is recomputed once
-// and its step-start states kept, so the reverse sweep costs one extra
-// forward run plus, per step, three stage recomputations and four
-// residual adjoints.
+// because every viscous and BR2 term is linear in mu.  Schedules: with
+// memory to spare, the forward sweep writes every stage state and its trace
+// records (and f) into one store and the reverse sweep only applies the four
+// residual adjoints per step; otherwise checkpoints every `interval` steps
+// (default ceil(sqrt(N))), each segment recomputed once with its step-start
+// states kept, so the reverse sweep costs one extra forward run plus, per
+// step, three stage recomputations and four residual adjoints.  Both run the
+// same kernels on the same inputs, so their gradients agree bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -193,23 +196,10 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
     return afail(c, DGX_ERR_UNSUPPORTED, "the discrete adjoint needs DGX_PREC_FP64");
   if (dJdmu && !(c->phys.mu > 0.0))
     return afail(c, DGX_ERR_INVALID, "dJ/dmu needs mu > 0");
-  const uint64_t K = interval ? interval : (uint64_t)std::ceil(std::sqrt((double)nsteps));
-  const uint64_t nchk = (nsteps + K - 1) / K;
   const size_t n = c->nstate;
   const size_t ntr = trace_doubles(c->p, c->nx, c->ny, c->nz);
 
-  std::vector<DevBuf> chk(nchk), seg(K);
-  for (auto& b : chk)
-    AHIP(c, alloc(b, n));
-  for (auto& b : seg)
-    AHIP(c, alloc(b, n));
-  DevBuf S[4], T[4], kf[4], lam, lk, acur, asum, tz, f0, dpart, dsum, final_state;
-  for (int i = 0; i < 4; ++i)
-  {
-    AHIP(c, alloc(S[i], n));
-    AHIP(c, alloc(T[i], ntr));
-    AHIP(c, alloc(kf[i], n));
-  }
+  DevBuf lam, lk, acur, asum, tz, f0, dpart, dsum, final_state;
   AHIP(c, alloc(lam, n));
   AHIP(c, alloc(lk, n));
   AHIP(c, alloc(acur, n));
@@ -228,80 +218,188 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   Phys ph0 = c->phys;
   ph0.mu = 0.0;
 
-  // forward sweep with checkpoints
-  for (uint64_t s = 0; s < nsteps; ++s)
+  // Schedule.  interval == 0: keep every stage state with its trace records
+  // (and f for dJ/dmu) from the forward sweep when that fits in three
+  // quarters of the free device memory, so the reverse sweep recomputes
+  // nothing; otherwise (or for interval >= 1) checkpoint every K steps.
+  const size_t per_step = 4 * (n + ntr) + (dJdmu ? 4 * n : 0);
+  DevBuf store;
+  bool store_all = false;
+  if (interval == 0)
   {
-    if (s % K == 0)
-      AHIP(c, copy_state(c, chk[s / K].p, c->U));
-    AHIP(c, step_ctx(c, dt));
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess &&
+        (double)nsteps * per_step * sizeof(double) <= 0.75 * (double)fr)
+    {
+      store_all = alloc(store, nsteps * per_step) == hipSuccess;
+      if (!store_all)
+        (void)hipGetLastError();  // fall back to checkpointing
+    }
   }
-  AHIP(c, copy_state(c, final_state.p, c->U));
+  const uint64_t K = interval ? interval : (uint64_t)std::ceil(std::sqrt((double)nsteps));
+  auto Sp = [&](uint64_t s, int i) { return store.p + (s * 4 + i) * (n + ntr); };
+  auto Tp = [&](uint64_t s, int i) { return Sp(s, i) + n; };
+  auto Kp = [&](uint64_t s, int i) { return store.p + nsteps * 4 * (n + ntr) + (s * 4 + i) * n; };
+
+  // reverse of one step: stage states Si with records Ti and (dJ/dmu only)
+  // their right-hand sides Ki = f(S_i); lam, the adjoint of the step's end
+  // state, becomes the adjoint of its start state
+  std::vector<double> dmu_parts;
+  auto reverse_step = [&](double* const* Si, double* const* Ti, double* const* Ki) -> dgx_status {
+    AHIP(c, copy_state(c, asum.p, lam.p));
+    for (int i = 3; i >= 0; --i)
+    {
+      // lk = w_l l + w_a a_(i+1) (a_(i+1) in acur)
+      const double wl = (i == 3 || i == 0) ? dt / 6.0 : dt / 3.0;
+      const double wa = (i == 2) ? dt : 0.5 * dt;
+      AHIP(c, copy_state(c, lk.p, lam.p));
+      if (i == 3)
+        AHIP(c, axpby_dispatch(lk.p, lk.p, 0.0, wl, n, c->stream));
+      else
+        AHIP(c, axpby_dispatch(lk.p, acur.p, wa, wl, n, c->stream));
+      if (dJdmu)
+      {
+        // df/dmu at S_i: (k_(i+1) - f(S_i; 0)) / mu
+        AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), ph0, Si[i], tz.p, c->status,
+                               c->stream));
+        AHIP(c, stage(c, ph0, Si[i], tz.p, nullptr, nullptr, nullptr, f0.p, 0.0));
+        AHIP(c, axpby_dispatch(f0.p, Ki[i], 1.0 / c->phys.mu, -1.0 / c->phys.mu, n, c->stream));
+        AHIP(c, dot_dispatch(lk.p, f0.p, n, dpart.p, ndot, dsum.p, c->stream));
+        double d = 0.0;
+        AHIP(c, hipMemcpyAsync(&d, dsum.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        AHIP(c, hipStreamSynchronize(c->stream));
+        dmu_parts.push_back(d);
+      }
+      AHIP(c, adjoint_dispatch(c->p, c->q, mesh_args(c), c->phys, Si[i], Ti[i], c->lam, lk.p,
+                               scr.pvol.p, scr.aown.p, scr.anbr.p, acur.p, c->stream));
+      AHIP(c, axpby_dispatch(asum.p, acur.p, 1.0, 1.0, n, c->stream));
+    }
+    AHIP(c, copy_state(c, lam.p, asum.p));
+    return DGX_OK;
+  };
+
+  const double cstage[4] = {0.5 * dt, 0.5 * dt, dt, dt};
+  const uint64_t nchk = store_all ? 0 : (nsteps + K - 1) / K;
+  std::vector<DevBuf> chk(nchk);
+  for (auto& b : chk)
+    AHIP(c, alloc(b, n));
+
+  // forward sweep
+  if (store_all)
+  {
+    // every stage state and its records written into the store: the stage
+    // arguments of stage_plan (RK4) with other output buffers
+    AHIP(c, copy_state(c, Sp(0, 0), c->U));
+    AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys, Sp(0, 0), Tp(0, 0), c->status,
+                           c->stream));
+    const int accmode[4] = {1, 2, 2, 3};
+    for (uint64_t s = 0; s < nsteps; ++s)
+      for (int i = 0; i < 4; ++i)
+      {
+        StageArgs sa;
+        sa.Vin = Sp(s, i);
+        sa.Tin = Tp(s, i);
+        sa.Vbase = Sp(s, 0);
+        sa.acc = c->acc;
+        if (i < 3)
+        {
+          sa.Vout = Sp(s, i + 1);
+          sa.Tout = Tp(s, i + 1);
+        }
+        else
+        {
+          sa.Vout = (s + 1 < nsteps) ? Sp(s + 1, 0) : final_state.p;
+          sa.Tout = (s + 1 < nsteps) ? Tp(s + 1, 0) : nullptr;
+        }
+        sa.fout = dJdmu ? Kp(s, i) : nullptr;
+        sa.ca = 1.0;
+        sa.cb = 0.0;
+        sa.cf = cstage[i];
+        sa.accmode = accmode[i];
+        sa.trace32 = 0;
+        AHIP(c, rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status,
+                             c->stage_waves, c->stream));
+      }
+    AHIP(c, copy_state(c, c->U, final_state.p));
+    c->tu_valid = false;
+  }
+  else
+  {
+    for (uint64_t s = 0; s < nsteps; ++s)
+    {
+      if (s % K == 0)
+        AHIP(c, copy_state(c, chk[s / K].p, c->U));
+      AHIP(c, step_ctx(c, dt));
+    }
+    AHIP(c, copy_state(c, final_state.p, c->U));
+  }
+
   // objective and its gradient at the final state
-  AHIP(c, kinetic_energy_dispatch(c->p, c->q, mesh_args(c), c->U, lam.p, c->partial, c->stream));
+  AHIP(c, kinetic_energy_dispatch(c->p, c->q, mesh_args(c), final_state.p, lam.p, c->partial,
+                                  c->stream));
   AHIP(c, sum_dispatch(c->partial, c->npartial, c->sum, c->stream));
   double Jv = 0.0;
   AHIP(c, hipMemcpyAsync(&Jv, c->sum, sizeof(double), hipMemcpyDeviceToHost, c->stream));
 
-  double dmu_total = 0.0;
-  std::vector<double> dmu_parts;
-  const double cstage[3] = {0.5 * dt, 0.5 * dt, dt};
-  for (uint64_t sg = nchk; sg-- > 0;)
+  // reverse sweep
+  if (store_all)
   {
-    const uint64_t s0 = sg * K, s1 = (s0 + K < nsteps) ? s0 + K : nsteps;
-    // recompute the segment's step-start states
-    AHIP(c, copy_state(c, c->U, chk[sg].p));
-    c->tu_valid = false;
-    for (uint64_t s = s0; s < s1; ++s)
+    for (uint64_t s = nsteps; s-- > 0;)
     {
-      AHIP(c, copy_state(c, seg[s - s0].p, c->U));
-      if (s + 1 < s1)
-        AHIP(c, step_ctx(c, dt));
-    }
-    for (uint64_t s = s1; s-- > s0;)
-    {
-      // stage states S0..S3 with their trace records and rhs k1..k4
-      AHIP(c, copy_state(c, S[0].p, seg[s - s0].p));
-      AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys, S[0].p, T[0].p, c->status,
-                             c->stream));
-      for (int i = 0; i < 3; ++i)
-        AHIP(c, stage(c, c->phys, S[i].p, T[i].p, S[0].p, S[i + 1].p, T[i + 1].p, kf[i].p,
-                      cstage[i]));
-      if (dJdmu)  // k4 only enters df/dmu
-        AHIP(c, stage(c, c->phys, S[3].p, T[3].p, nullptr, nullptr, nullptr, kf[3].p, 0.0));
-      // reverse: l_(i+1) weights of k_(i+1), a_i adjoint through S_i
-      AHIP(c, copy_state(c, asum.p, lam.p));
-      for (int i = 3; i >= 0; --i)
-      {
-        // lk = w_l l + w_a a_(i+1) (a_(i+1) in acur)
-        const double wl = (i == 3 || i == 0) ? dt / 6.0 : dt / 3.0;
-        const double wa = (i == 2) ? dt : 0.5 * dt;
-        AHIP(c, copy_state(c, lk.p, lam.p));
-        if (i == 3)
-          AHIP(c, axpby_dispatch(lk.p, lk.p, 0.0, wl, n, c->stream));
-        else
-          AHIP(c, axpby_dispatch(lk.p, acur.p, wa, wl, n, c->stream));
-        if (dJdmu)
-        {
-          // df/dmu at S_i: (k_(i+1) - f(S_i; 0)) / mu
-          AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), ph0, S[i].p, tz.p, c->status,
-                                 c->stream));
-          AHIP(c, stage(c, ph0, S[i].p, tz.p, nullptr, nullptr, nullptr, f0.p, 0.0));
-          AHIP(c, axpby_dispatch(f0.p, kf[i].p, 1.0 / c->phys.mu, -1.0 / c->phys.mu, n,
-                                 c->stream));
-          AHIP(c, dot_dispatch(lk.p, f0.p, n, dpart.p, ndot, dsum.p, c->stream));
-          double d = 0.0;
-          AHIP(c, hipMemcpyAsync(&d, dsum.p, sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
-          AHIP(c, hipStreamSynchronize(c->stream));
-          dmu_parts.push_back(d);
-        }
-        AHIP(c, adjoint_dispatch(c->p, c->q, mesh_args(c), c->phys, S[i].p, T[i].p, c->lam,
-                                 lk.p, scr.pvol.p, scr.aown.p, scr.anbr.p, acur.p, c->stream));
-        AHIP(c, axpby_dispatch(asum.p, acur.p, 1.0, 1.0, n, c->stream));
-      }
-      AHIP(c, copy_state(c, lam.p, asum.p));
+      double* const Si[4] = {Sp(s, 0), Sp(s, 1), Sp(s, 2), Sp(s, 3)};
+      double* const Ti[4] = {Tp(s, 0), Tp(s, 1), Tp(s, 2), Tp(s, 3)};
+      double* const Ki[4] = {dJdmu ? Kp(s, 0) : nullptr, dJdmu ? Kp(s, 1) : nullptr,
+                             dJdmu ? Kp(s, 2) : nullptr, dJdmu ? Kp(s, 3) : nullptr};
+      dgx_status st = reverse_step(Si, Ti, Ki);
+      if (st != DGX_OK)
+        return st;
     }
   }
+  else
+  {
+    // per segment: recompute its step-start states, then per step rebuild
+    // the stage states S0..S3 with their records and rhs k1..k4, reverse it
+    std::vector<DevBuf> seg(K);
+    for (auto& b : seg)
+      AHIP(c, alloc(b, n));
+    DevBuf S[4], T[4], kf[4];
+    for (int i = 0; i < 4; ++i)
+    {
+      AHIP(c, alloc(S[i], n));
+      AHIP(c, alloc(T[i], ntr));
+      AHIP(c, alloc(kf[i], n));
+    }
+    double* const Si[4] = {S[0].p, S[1].p, S[2].p, S[3].p};
+    double* const Ti[4] = {T[0].p, T[1].p, T[2].p, T[3].p};
+    double* const Ki[4] = {kf[0].p, kf[1].p, kf[2].p, kf[3].p};
+    for (uint64_t sg = nchk; sg-- > 0;)
+    {
+      const uint64_t s0 = sg * K, s1 = (s0 + K < nsteps) ? s0 + K : nsteps;
+      AHIP(c, copy_state(c, c->U, chk[sg].p));
+      c->tu_valid = false;
+      for (uint64_t s = s0; s < s1; ++s)
+      {
+        AHIP(c, copy_state(c, seg[s - s0].p, c->U));
+        if (s + 1 < s1)
+          AHIP(c, step_ctx(c, dt));
+      }
+      for (uint64_t s = s1; s-- > s0;)
+      {
+        AHIP(c, copy_state(c, S[0].p, seg[s - s0].p));
+        AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys, S[0].p, T[0].p, c->status,
+                               c->stream));
+        for (int i = 0; i < 3; ++i)
+          AHIP(c, stage(c, c->phys, S[i].p, T[i].p, S[0].p, S[i + 1].p, T[i + 1].p, kf[i].p,
+                        cstage[i]));
+        if (dJdmu)  // k4 only enters df/dmu
+          AHIP(c, stage(c, c->phys, S[3].p, T[3].p, nullptr, nullptr, nullptr, kf[3].p, 0.0));
+        dgx_status st = reverse_step(Si, Ti, Ki);
+        if (st != DGX_OK)
+          return st;
+      }
+    }
+  }
+  double dmu_total = 0.0;
   for (size_t i = dmu_parts.size(); i-- > 0;)
     dmu_total += dmu_parts[i];
   // gradient with respect to the initial state in the reference layout

@@ -119,6 +119,208 @@ __device__ __forceinline__ void roe_flux(const T* Ul, const T* Ur, const double*
   f[4] = 0.5 * ((mnl * Hl) + (mnr * Hr)) - 0.5 * (al3 * drE + C1 * H + C2 * un);
 }
 
+// Reverse mode of roe_flux: gl = d(psi . f)/dUl and gr = d(psi . f)/dUr,
+// the transposed Roe flux Jacobians applied to psi.  One forward sweep keeps
+// the intermediates, one backward sweep accumulates their adjoints (bX is
+// the adjoint of X); the entropy fix is differentiated on the branch the
+// forward sweep takes, and |l| with derivative +1 at 0, as the dual numbers
+// do.
+__device__ __forceinline__ void roe_flux_vjp(const double* Ul, const double* Ur, const double* n,
+                                             double g, const double* psi, double* gl,
+                                             double* gr)
+{
+  // ---- forward sweep ----
+  const double gm1 = g - 1.0;
+  const double irl = rcp(Ul[0]), irr = rcp(Ur[0]);
+  const double vl[3] = {Ul[1] * irl, Ul[2] * irl, Ul[3] * irl};
+  const double vr[3] = {Ur[1] * irr, Ur[2] * irr, Ur[3] * irr};
+  const double ql = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2];
+  const double qr = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2];
+  const double pl = gm1 * (Ul[4] - 0.5 * Ul[0] * ql);
+  const double pr = gm1 * (Ur[4] - 0.5 * Ur[0] * qr);
+  const double Hl = (Ul[4] + pl) * irl, Hr = (Ur[4] + pr) * irr;
+  const double sl = sqrt(Ul[0]), sr = sqrt(Ur[0]);
+  const double iden = rcp(sl + sr);
+  double va[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    va[k] = (sl * vl[k] + sr * vr[k]) * iden;
+  const double H  = (sl * Hl + sr * Hr) * iden;
+  const double dr = Ur[0] - Ul[0], drE = Ur[4] - Ul[4];
+  const double dm[3] = {Ur[1] - Ul[1], Ur[2] - Ul[2], Ur[3] - Ul[3]};
+  const double qsq = va[0] * va[0] + va[1] * va[1] + va[2] * va[2];
+  const double c   = sqrt(gm1 * (H - 0.5 * qsq));
+  const double ic  = rcp(c);
+  const double un  = va[0] * n[0] + va[1] * n[1] + va[2] * n[2];
+  const double lk[3] = {un + c, un - c, un};
+  const double eps = 0.05 * c;
+  const double i2e = rcp(2.0 * eps);
+  double al[3];
+  bool fix[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    al[k]  = fabs(lk[k]);
+    fix[k] = al[k] < eps;
+    if (fix[k])
+      al[k] = fabs((eps * eps + lk[k] * lk[k]) * i2e);
+  }
+  const double s1 = 0.5 * (al[0] + al[1]), s2 = 0.5 * (al[0] - al[1]);
+  const double udm = va[0] * dm[0] + va[1] * dm[1] + va[2] * dm[2];
+  const double G1 = gm1 * (0.5 * qsq * dr - udm + drE);
+  const double G2 = -un * dr + (dm[0] * n[0] + dm[1] * n[1] + dm[2] * n[2]);
+  const double s13 = s1 - al[2];
+  const double C1 = (G1 * ic * ic) * s13 + (G2 * ic) * s2;
+  const double C2 = (G1 * ic) * s2 + s13 * G2;
+  const double unl = vl[0] * n[0] + vl[1] * n[1] + vl[2] * n[2];
+  const double unr = vr[0] * n[0] + vr[1] * n[1] + vr[2] * n[2];
+  const double mnl = Ul[0] * unl, mnr = Ur[0] * unr;
+
+  // ---- backward sweep ----
+  double bvl[3] = {0.0, 0.0, 0.0}, bvr[3] = {0.0, 0.0, 0.0}, bva[3] = {0.0, 0.0, 0.0};
+  double bdm[3] = {0.0, 0.0, 0.0};
+  double bmnl = 0.5 * psi[0], bmnr = 0.5 * psi[0];
+  double bal3 = -0.5 * psi[0] * dr, bdr = -0.5 * psi[0] * al[2], bC1 = -0.5 * psi[0];
+  double bpl = 0.0, bpr = 0.0, bC2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    const double ps = psi[1 + k];
+    bmnl += 0.5 * ps * vl[k];
+    bvl[k] += 0.5 * ps * mnl;
+    bpl += 0.5 * ps * n[k];
+    bmnr += 0.5 * ps * vr[k];
+    bvr[k] += 0.5 * ps * mnr;
+    bpr += 0.5 * ps * n[k];
+    bal3 -= 0.5 * ps * dm[k];
+    bdm[k] -= 0.5 * ps * al[2];
+    bC1 -= 0.5 * ps * va[k];
+    bva[k] -= 0.5 * ps * C1;
+    bC2 -= 0.5 * ps * n[k];
+  }
+  const double p4 = psi[4];
+  bmnl += 0.5 * p4 * Hl;
+  double bHl = 0.5 * p4 * mnl;
+  bmnr += 0.5 * p4 * Hr;
+  double bHr = 0.5 * p4 * mnr;
+  bal3 -= 0.5 * p4 * drE;
+  double bdrE = -0.5 * p4 * al[2];
+  bC1 -= 0.5 * p4 * H;
+  double bH  = -0.5 * p4 * C1;
+  bC2 -= 0.5 * p4 * un;
+  double bun = -0.5 * p4 * C2;
+  // mnl = rhol unl, unl = vl . n (and the right side)
+  double brl = bmnl * unl, brr = bmnr * unr;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    bvl[k] += bmnl * Ul[0] * n[k];
+    bvr[k] += bmnr * Ur[0] * n[k];
+  }
+  // C2 = G1 ic s2 + s13 G2,  C1 = G1 ic^2 s13 + G2 ic s2
+  double bG1 = bC2 * ic * s2 + bC1 * ic * ic * s13;
+  double bG2 = bC2 * s13 + bC1 * ic * s2;
+  double bic = bC2 * G1 * s2 + bC1 * (2.0 * G1 * ic * s13 + G2 * s2);
+  double bs2 = bC2 * G1 * ic + bC1 * G2 * ic;
+  double bs13 = bC2 * G2 + bC1 * G1 * ic * ic;
+  double bs1 = bs13;
+  bal3 -= bs13;
+  // G2 = -un dr + dm . n
+  bun -= bG2 * dr;
+  bdr -= bG2 * un;
+  // G1 = gm1 (qsq dr / 2 - va . dm + drE)
+  double bqsq = bG1 * gm1 * 0.5 * dr;
+  bdr += bG1 * gm1 * 0.5 * qsq;
+  bdrE += bG1 * gm1;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    bdm[k] += bG2 * n[k] - bG1 * gm1 * va[k];
+    bva[k] -= bG1 * gm1 * dm[k];
+  }
+  // s1, s2 from al1, al2; entropy fix; l1 = un + c, l2 = un - c, l3 = un
+  const double bal[3] = {0.5 * (bs1 + bs2), 0.5 * (bs1 - bs2), bal3};
+  double bc = 0.0, beps = 0.0;
+  double bl[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    if (fix[k])
+    {
+      const double ie = 2.0 * i2e;  // 1 / eps
+      bl[k] = bal[k] * lk[k] * ie;
+      beps += bal[k] * (0.5 - 0.5 * lk[k] * lk[k] * ie * ie);
+    }
+    else
+      bl[k] = (lk[k] < 0.0) ? -bal[k] : bal[k];
+  }
+  bun += bl[0] + bl[1] + bl[2];
+  bc += bl[0] - bl[1] + 0.05 * beps;
+  // ic = 1/c;  c = sqrt(gm1 (H - qsq/2))
+  bc -= bic * ic * ic;
+  bH += bc * gm1 * 0.5 * ic;
+  bqsq -= bc * gm1 * 0.25 * ic;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    bva[k] += bun * n[k] + 2.0 * va[k] * bqsq;
+  // Roe averages: va = (sl vl + sr vr) iden, H = (sl Hl + sr Hr) iden
+  double bsl = bH * Hl * iden, bsr = bH * Hr * iden;
+  double biden = bH * (sl * Hl + sr * Hr);
+  bHl += bH * sl * iden;
+  bHr += bH * sr * iden;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    bsl += bva[k] * vl[k] * iden;
+    bsr += bva[k] * vr[k] * iden;
+    biden += bva[k] * (sl * vl[k] + sr * vr[k]);
+    bvl[k] += bva[k] * sl * iden;
+    bvr[k] += bva[k] * sr * iden;
+  }
+  bsl -= biden * iden * iden;
+  bsr -= biden * iden * iden;
+  brl += bsl * 0.5 * rcp(sl);
+  brr += bsr * 0.5 * rcp(sr);
+  // H_side = (E + p) ir;  p = gm1 (E - rho q / 2)
+  double bEl = bHl * irl, bEr = bHr * irr;
+  bpl += bHl * irl;
+  bpr += bHr * irr;
+  double birl = bHl * (Ul[4] + pl), birr = bHr * (Ur[4] + pr);
+  bEl += gm1 * bpl;
+  bEr += gm1 * bpr;
+  brl -= 0.5 * gm1 * bpl * ql;
+  brr -= 0.5 * gm1 * bpr * qr;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    bvl[k] -= gm1 * bpl * Ul[0] * vl[k];
+    bvr[k] -= gm1 * bpr * Ur[0] * vr[k];
+  }
+  // v = m ir;  ir = 1/rho
+  double bml[3], bmr[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    bml[k] = bvl[k] * irl;
+    bmr[k] = bvr[k] * irr;
+    birl += bvl[k] * Ul[1 + k];
+    birr += bvr[k] * Ur[1 + k];
+  }
+  brl -= birl * irl * irl;
+  brr -= birr * irr * irr;
+  // differences dr, dm, drE (Ur - Ul)
+  gl[0] = brl - bdr;
+  gr[0] = brr + bdr;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    gl[1 + k] = bml[k] - bdm[k];
+    gr[1 + k] = bmr[k] + bdm[k];
+  }
+  gl[4] = bEl - bdrE;
+  gr[4] = bEr + bdrE;
+}
+
 // State-dependent part of the Newtonian viscous flux A(s, grad): the
 // velocity and 1/rho.  A is linear in its gradient argument.
 struct ViscState

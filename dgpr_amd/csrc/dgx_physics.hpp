@@ -1,6 +1,9 @@
 // dgx_physics.hpp -- pointwise physics of the stage kernel written once for
 // a generic scalar type T: double in the residual kernels, forward-mode dual
-// numbers (local Jacobians) in the discrete adjoint.
+// numbers (local Jacobians) in the discrete adjoint.  Arguments that the
+// adjoint's dual passes hold fixed (reference-space gradients, gradient
+// tractions) have their own type G / V, so a pass over the state carries no
+// zero derivatives for them.
 #pragma once
 
 #include "dgx_device.hpp"
@@ -30,8 +33,8 @@ __device__ __forceinline__ void tau_n(double mu, const T* xn, const T* xtn, T tr
 
 // velocity gradient from reference-space derivatives gr[c][0..3] of
 // (rho, m) and the inverse Jacobian Ji[c][d]
-template <typename T>
-__device__ __forceinline__ void vel_grad(const T gr[3][4], const T* u, T ir, const double Ji[3][3],
+template <typename T, typename G>
+__device__ __forceinline__ void vel_grad(const G gr[3][4], const T* u, T ir, const double Ji[3][3],
                                          T Lv[3][3])
 {
   T dv[3][3];  // dv[c][i] = d u_i / d xi_c
@@ -72,8 +75,8 @@ using FacePoint = FacePointT<double>;
 // derivatives gr[c][0..3] of (rho, m), the inverse Jacobian Ji and the normal:
 // with dv[c][i] = d u_i/d xi_c,  (L n)_i = sum_c dv[c][i] (Ji n)_c,
 // (L^T n)_d = sum_c Ji[c][d] (dv[c] . n),  tr L = sum_c dv[c] . Ji[c].
-template <typename T>
-__device__ __forceinline__ void grad_traction(const T gr[3][4], const T* U, const double Ji[3][3],
+template <typename T, typename G>
+__device__ __forceinline__ void grad_traction(const G gr[3][4], const T* U, const double Ji[3][3],
                                               const double* n, double mu, T* vg)
 {
   T ir = rcp(U[0]);
@@ -101,9 +104,9 @@ __device__ __forceinline__ void grad_traction(const T gr[3][4], const T* U, cons
 // vgL, vgR: gradient tractions tau(grad u).n of the own and neighbour traces
 // (vgR unused on boundary faces).  tau(a (x) n).n = mu a + mu/3 (a.n) n.
 // Ub is the neighbour trace (interior) or the boundary state (boundary).
-template <typename T>
+template <typename T, typename V>
 __device__ __forceinline__ void face_point_physics(bool interior, const T* UL, const T* Ub,
-                                                   const T* vgL, const T* vgR, const double* n,
+                                                   const V* vgL, const V* vgR, const double* n,
                                                    double lamL, double lamR, const Phys& ph,
                                                    FacePointT<T>& out)
 {
@@ -169,8 +172,8 @@ __device__ __forceinline__ void face_point_physics(bool interior, const T* UL, c
 // Volume flux H = Q - F (flux.cpp:24-48 Euler, flux.cpp:156-202 Newtonian
 // viscous, no heat flux) at a Gauss node, in velocity-gradient form, from the
 // node state Vq and the reference-space derivatives gr[c][0..3] of (rho, m).
-template <typename T>
-__device__ __forceinline__ void volume_flux(const T* Vq, const T gr[3][4], const double Ji[3][3],
+template <typename T, typename G>
+__device__ __forceinline__ void volume_flux(const T* Vq, const G gr[3][4], const double Ji[3][3],
                                             double gamma, double mu, T H[3][5])
 {
   T ir = rcp(Vq[0]);

@@ -241,8 +241,14 @@ dgx_status dgx_residual_vjp(dgx_ctx* ctx, const double* w_host, double* g_host);
 /* Runs nsteps classic RK4 steps of size dt from the context's state (which
  * ends at the final state, as after dgx_step) and returns the final kinetic
  * energy J, its gradient with respect to the initial state (simstate layout,
- * may be NULL) and dJ/dmu (may be NULL), by the discrete adjoint with a
- * checkpoint every `interval` steps (0: ceil(sqrt(nsteps))). */
+ * may be NULL) and dJ/dmu (may be NULL), by the discrete adjoint.
+ * interval >= 1: a checkpoint every `interval` steps, each segment
+ * recomputed once in the reverse sweep.  interval = 0: every stage state
+ * with its trace records (and f, for dJ/dmu) is kept from the forward sweep
+ * when nsteps * 4 * (state + records [+ state]) fits in three quarters of
+ * the free device memory, so nothing is recomputed; otherwise checkpoints
+ * every ceil(sqrt(nsteps)) steps.  Both schedules give the same gradients
+ * bit for bit. */
 dgx_status dgx_adjoint_rk4(dgx_ctx* ctx, double dt, uint64_t nsteps, uint64_t interval,
                            double* J, double* dJdU0_host, double* dJdmu);
 

@@ -93,6 +93,23 @@ def test_adjoint_rk4_gradient_matches_fd(built, name, nsteps, interval):
     assert err < FD_TOL
 
 
+@pytest.mark.parametrize("name", ["tgv2_p2_curved", "tgv3_p1"])
+def test_adjoint_schedules_agree_bitwise(built, name):
+    """Stored stages (interval 0, which fits here), every step-start state
+    kept (1) and checkpoints every 3 steps run the same kernels on the same
+    inputs: J, both gradients and the final state agree bit for bit."""
+    c = small_cases()[name]
+    out = []
+    for interval in (0, 1, 3):
+        with dgx.Solver(c.mesh, c.params) as s:
+            s.set_state(c.U0)
+            J, g, dmu = s.adjoint_rk4(c.dt, 5, interval)
+            out.append((J, g, dmu, s.get_state()))
+    for J, g, dmu, U in out[1:]:
+        assert J == out[0][0] and dmu == out[0][2]
+        assert np.array_equal(g, out[0][1]) and np.array_equal(U, out[0][3])
+
+
 def test_adjoint_needs_periodic_mesh(built):
     c = cases.channel_case(3, 3, 2, p=2, seed=16)
     with dgx.Solver(c.mesh, c.params) as s:
