@@ -1,8 +1,9 @@
-// dgx_dual.hpp -- forward-mode dual numbers for the local Jacobians of the
-// discrete adjoint: a value with D directional derivatives.  The pointwise
-// physics (dgx_physics.hpp) is written for a generic scalar type, so running
-// it on Dual<D> inputs seeded with unit directions gives D columns of its
-// Jacobian at once.
+// dgx_dual.hpp -- forward-mode dual numbers: a value with D directional
+// derivatives.  The pointwise physics (dgx_physics.hpp) is written for a
+// generic scalar type, so running it on Dual<D> inputs seeded with unit
+// directions gives D columns of its Jacobian at once; the reverse sweeps of
+// the discrete adjoint (dgx_vjp.hpp, roe_flux_vjp) are checked against it
+// (tests/cpu/vjp_check.cpp).  Not used by the kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>

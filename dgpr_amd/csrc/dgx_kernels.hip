@@ -28,8 +28,8 @@
 // neighbour's gradient.
 #include <hip/hip_runtime.h>
 
-#include "dgx_dual.hpp"
 #include "dgx_physics.hpp"
+#include "dgx_vjp.hpp"
 
 namespace dgx
 {
