@@ -8,16 +8,20 @@
 // from its own side with both traces from the trace records, gather,
 // f_g = s - acc / wJ) is reversed step by step:
 //   A: volume terms (transposed contractions, local Jacobian of the volume
-//      flux) into pvol; each face point's physics reversed into the adjoint
-//      of the element's own trace record (aown) and of its neighbour's record
-//      (anbr, written into the neighbour's slot: every slot has one writer);
+//      flux) into pvol, then, in a second launch, each face point's physics
+//      reversed into the adjoint of the element's own trace record (aown) and
+//      of its neighbour's record (anbr, written into the neighbour's slot:
+//      every slot has one writer);
 //   B: both adjoint records of each face point reversed through the trace
 //      production (trace interpolation, tangential derivatives, gradient
 //      traction), added to pvol.
 // The local Jacobians of the pointwise physics are the reverse sweeps of
 // dgx_vjp.hpp and roe_flux_vjp (dgx_device.hpp).
 
-template <int P, int Q>
+// PART 0: the volume terms (into pvol); PART 1: the face terms (into aown,
+// anbr).  Separate launches: the volume part needs about half the registers
+// of the face part and runs at a higher occupancy on its own.
+template <int P, int Q, int PART>
 __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph,
                                                           const double* __restrict__ V,
                                                           const double* __restrict__ Tin,
@@ -43,7 +47,8 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
   const int tid = threadIdx.x, le = tid / NB, nd0 = tid % NB;
   const int e0 = blockIdx.x * EPW, e = e0 + le;
   const bool vact = (le < EPW) && (e < mesh.nelem);
-  face_table<EPW>(mesh, e0, sNb, sBt);
+  if constexpr (PART == 1)
+    face_table<EPW>(mesh, e0, sNb, sBt);
   for (int x = tid; x < EPW * NG * 3; x += 64)
   {
     int ee = e0 + x / (NG * 3);
@@ -52,7 +57,7 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
   }
   const int idx[3] = {nd0 % N, (nd0 / N) % N, nd0 / (N * N)};
   double Ji[3][3], wJ = 1.0;
-  if (vact)
+  if (PART == 0 && vact)
   {
 #pragma unroll
     for (int r = 0; r < 5; ++r)
@@ -73,6 +78,8 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
   }
   __syncthreads();
 
+  if constexpr (PART == 0)
+  {
   // ---- volume terms ----
   double pv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   double pgr[3][4];
@@ -168,6 +175,8 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
     for (int r = 0; r < 5; ++r)
       pvol[((size_t)e * 5 + r) * NB + nd0] = pv[r];
   }
+  return;
+  }
 
   // ---- faces ----
   for (int pass = 0; pass < FD::NPASS; ++pass)
@@ -249,12 +258,11 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
       const double fJ = face_normal(J, nd, hi, nv);
       const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
       const double lamR = lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
-      FacePoint fpo;
-      face_point_physics(true, UL, UR, vgL, vgR, nv, lamL, lamR, ph, fpo);
+      double S[6];
+      interior_dc_stress(UL, UR, nv, ph.mu, S);
       const double irL = rcp(UL[0]);
       const double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
       const double sc = fJ * pt.w[a] * pt.w[b];
-      const double* S = fpo.S;
       const double Sm[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
       double hc[3][3];
 #pragma unroll

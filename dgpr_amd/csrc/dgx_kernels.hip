@@ -1384,9 +1384,14 @@ hipError_t launch_adjoint(const MeshArgs& m, const Phys& ph, const double* V, co
 {
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
-  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, V, Tin,
+  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q, 0>), dim3(nblk), dim3(64), 0, st, m, ph, V, Tin,
                      lam, psi, pvol, aown, anbr);
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q, 1>), dim3(nblk), dim3(64), 0, st, m, ph, V, Tin,
+                     lam, psi, pvol, aown, anbr);
+  e = hipGetLastError();
   if (e != hipSuccess)
     return e;
   hipLaunchKernelGGL((dgx_adj_trace_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, V, aown,

@@ -11,6 +11,28 @@
 namespace dgx
 {
 
+// The dual-consistency stress S of an interior face point, the out.S of
+// face_point_physics(true, ...): tau(dmL/2 (x) n) with dmL the velocity-like
+// jump (m jump - u_L rho jump)/rho_L.  It needs neither the Roe flux nor the
+// tractions, so the adjoint kernel forms it on its own.
+__device__ __forceinline__ void interior_dc_stress(const double* UL, const double* UR,
+                                                   const double* n, double mu, double* S)
+{
+  const double irL = rcp(UL[0]);
+  const double jr  = UL[0] - UR[0];
+  double a[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    a[i] = 0.5 * ((UL[1 + i] - UR[1 + i]) - jr * (UL[1 + i] * irL)) * irL;
+  const double lt = -(2.0 / 3.0) * mu * (a[0] * n[0] + a[1] * n[1] + a[2] * n[2]);
+  S[0] = 2.0 * mu * a[0] * n[0] + lt;
+  S[1] = 2.0 * mu * a[1] * n[1] + lt;
+  S[2] = 2.0 * mu * a[2] * n[2] + lt;
+  S[3] = mu * (a[0] * n[1] + a[1] * n[0]);
+  S[4] = mu * (a[0] * n[2] + a[2] * n[0]);
+  S[5] = mu * (a[1] * n[2] + a[2] * n[1]);
+}
+
 // Volume flux H(Vq, gr) (dgx_physics.hpp): gx[0..4] = d/dVq, gx[5 + 4c + r]
 // = d/dgr[c][r] of sum psiH . H, in reverse mode.  Forward: u = m/rho,
 // dv[c][i] = (gr[c][1+i] - gr[c][0] u_i)/rho, Lv[i][d] = sum_c dv[c][i] Ji[c][d],

@@ -7,6 +7,7 @@
 //   volume_flux_vjp     vs  volume_flux<Dual<5>> (+ exact linear differences
 //                           for the reference-space gradients)
 //   traction_vjp        vs  grad_traction<Dual<4>> (+ the same for gradients)
+//   interior_dc_stress  vs  the S output of face_point_physics
 // Errors are relative to the largest entry of each gradient; the program
 // prints the worst per sweep and exits non-zero above 1e-12.
 #include <hip/hip_runtime.h>
@@ -143,6 +144,12 @@ int main()
           ref[10 + j] += psiS[k] * (f1.S[k] - f0.S[k]);
       }
       w_face = std::fmax(w_face, rel_worst(gx, ref, 16));
+      // the adjoint's own dual-consistency stress equals face_point_physics's
+      FacePoint fp;
+      face_point_physics(true, UL, UR, vgL, vgR, n, lamL, lamR, ph, fp);
+      double S[6];
+      interior_dc_stress(UL, UR, n, ph.mu, S);
+      w_face = std::fmax(w_face, rel_worst(S, fp.S, 6));
     }
     // volume flux and gradient traction at a node
     {
