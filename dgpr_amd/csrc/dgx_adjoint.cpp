@@ -29,6 +29,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "../../include/dgx.h"
@@ -245,18 +246,15 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   // their right-hand sides Ki = f(S_i); lam, the adjoint of the step's end
   // state, becomes the adjoint of its start state
   std::vector<double> dmu_parts;
+  // weights of stage k in the adjoint of V': l_k = w_l(k) l + w_a(k) a_(k+1)
+  auto wl_of = [&](int k) { return (k == 3 || k == 0) ? dt / 6.0 : dt / 3.0; };
+  auto wa_of = [&](int k) { return (k == 2) ? dt : 0.5 * dt; };
   auto reverse_step = [&](double* const* Si, double* const* Ti, double* const* Ki) -> dgx_status {
-    AHIP(c, copy_state(c, asum.p, lam.p));
+    // lk of stage 3 = w_l l
+    AHIP(c, adj_tail_dispatch(lam.p, nullptr, nullptr, nullptr, lk.p, wl_of(3), 0.0, n,
+                              c->stream));
     for (int i = 3; i >= 0; --i)
     {
-      // lk = w_l l + w_a a_(i+1) (a_(i+1) in acur)
-      const double wl = (i == 3 || i == 0) ? dt / 6.0 : dt / 3.0;
-      const double wa = (i == 2) ? dt : 0.5 * dt;
-      AHIP(c, copy_state(c, lk.p, lam.p));
-      if (i == 3)
-        AHIP(c, axpby_dispatch(lk.p, lk.p, 0.0, wl, n, c->stream));
-      else
-        AHIP(c, axpby_dispatch(lk.p, acur.p, wa, wl, n, c->stream));
       if (dJdmu)
       {
         // df/dmu at S_i: (k_(i+1) - f(S_i; 0)) / mu
@@ -272,9 +270,12 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
       }
       AHIP(c, adjoint_dispatch(c->p, c->q, mesh_args(c), c->phys, Si[i], Ti[i], c->lam, lk.p,
                                scr.pvol.p, scr.aown.p, scr.anbr.p, acur.p, c->stream));
-      AHIP(c, axpby_dispatch(asum.p, acur.p, 1.0, 1.0, n, c->stream));
+      // asum = l + a_3 + ... + a_i; lk of stage i-1 = w_l l + w_a a_i
+      AHIP(c, adj_tail_dispatch(lam.p, acur.p, (i == 3) ? lam.p : asum.p, asum.p,
+                                (i > 0) ? lk.p : nullptr, (i > 0) ? wl_of(i - 1) : 0.0,
+                                (i > 0) ? wa_of(i - 1) : 0.0, n, c->stream));
     }
-    AHIP(c, copy_state(c, lam.p, asum.p));
+    std::swap(lam.p, asum.p);  // the adjoint of the step's start state becomes l
     return DGX_OK;
   };
 

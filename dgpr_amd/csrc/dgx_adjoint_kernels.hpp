@@ -568,6 +568,31 @@ __global__ __launch_bounds__(256) void dgx_axpby_kernel(double* __restrict__ y,
     y[i] = a * x[i] + b * y[i];
 }
 
+// Tail of one reverse RK stage, elementwise in one pass: with the stage's
+// adjoint a, sum_out = sum_in + a (sum_in may be sum_out) and the next
+// stage's input lk = wa a + wl l; without a, lk = wl l.  Null outputs are
+// skipped.
+__global__ __launch_bounds__(256) void dgx_adj_tail_kernel(const double* __restrict__ l,
+                                                           const double* __restrict__ a,
+                                                           const double* sum_in, double* sum_out,
+                                                           double* __restrict__ lk, double wl,
+                                                           double wa, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  if (a)
+  {
+    const double ai = a[i];
+    if (sum_out)
+      sum_out[i] = sum_in[i] + ai;
+    if (lk)
+      lk[i] = wa * ai + wl * l[i];
+  }
+  else if (lk)
+    lk[i] = wl * l[i];
+}
+
 // per-block partial sums of x . y (256 threads, grid-stride)
 __global__ __launch_bounds__(256) void dgx_dot_kernel(const double* __restrict__ x,
                                                       const double* __restrict__ y, size_t n,

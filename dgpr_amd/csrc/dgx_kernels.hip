@@ -1436,6 +1436,16 @@ hipError_t axpby_dispatch(double* y, const double* x, double a, double b, size_t
   return hipGetLastError();
 }
 
+hipError_t adj_tail_dispatch(const double* l, const double* a, const double* sum_in,
+                             double* sum_out, double* lk, double wl, double wa, size_t n,
+                             hipStream_t st)
+{
+  int nblk = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(dgx_adj_tail_kernel, dim3(nblk), dim3(256), 0, st, l, a, sum_in, sum_out,
+                     lk, wl, wa, n);
+  return hipGetLastError();
+}
+
 hipError_t dot_dispatch(const double* x, const double* y, size_t n, double* partial,
                         int nblocks, double* out, hipStream_t st)
 {

@@ -121,6 +121,11 @@ hipError_t adjoint_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, con
 // gradient with respect to the Gauss-nodal state
 hipError_t kinetic_energy_dispatch(int p, int q, const MeshArgs& m, const double* V,
                                    double* grad, double* partial, hipStream_t st);
+// one pass of the reverse RK stage algebra: sum_out = sum_in + a and
+// lk = wa a + wl l (a null: lk = wl l); null outputs skipped
+hipError_t adj_tail_dispatch(const double* l, const double* a, const double* sum_in,
+                             double* sum_out, double* lk, double wl, double wa, size_t n,
+                             hipStream_t st);
 // y = a x + b y
 hipError_t axpby_dispatch(double* y, const double* x, double a, double b, size_t n,
                           hipStream_t st);
