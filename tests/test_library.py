@@ -154,3 +154,26 @@ def test_create_rejects_bad_input():
     with pytest.raises(dgx.DgxError) as ei:
         dgx.Solver(hi, c.params)
     assert ei.value.status == 2
+
+
+def test_entry_points_reject_null_arguments():
+    """Every context-taking entry point returns DGX_ERR_INVALID for a null
+    context (or null required buffer) instead of touching memory; the
+    generators reject a null mesh."""
+    L = dgx.lib()
+    buf = (C.c_double * 8)()
+    i = C.c_int()
+    J = C.c_double()
+    assert L.dgx_set_precision(None, dgx.PREC_MIXED) == dgx.ERR_INVALID
+    assert L.dgx_get_precision(None, C.byref(i)) == dgx.ERR_INVALID
+    assert L.dgx_set_tuning(None, 2) == dgx.ERR_INVALID
+    assert L.dgx_step(None, dgx.RK4, 1e-3, 1, None) == dgx.ERR_INVALID
+    assert L.dgx_kinetic_energy(None, C.byref(J)) == dgx.ERR_INVALID
+    assert L.dgx_residual_vjp(None, buf, buf) == dgx.ERR_INVALID
+    assert L.dgx_adjoint_rk4(None, 1e-3, 2, 0, C.byref(J), buf, None) == dgx.ERR_INVALID
+    assert L.dgx_turbulent_state(None, None, 1.0, buf) == dgx.ERR_INVALID
+    c = cases.tgv_case(n=2, p=1)
+    cm = c.mesh.cstruct()
+    prm = c.params.copy()
+    assert L.dgx_turbulent_state(C.byref(cm), C.byref(prm), 0.0, buf) == dgx.ERR_INVALID
+    assert L.dgx_turbulent_state(C.byref(cm), None, 1.0, buf) == dgx.ERR_INVALID
