@@ -213,7 +213,8 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   }
   const int ndot = 256;
   AHIP(c, alloc(dpart, ndot));
-  AHIP(c, alloc(dsum, 1));
+  // one dot product per reverse stage for dJ/dmu, read back once at the end
+  AHIP(c, alloc(dsum, dJdmu ? 4 * nsteps : 1));
   AdjScratch scr;
   AHIP(c, alloc_scratch(c, scr));
   Phys ph0 = c->phys;
@@ -245,7 +246,7 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   // reverse of one step: stage states Si with records Ti and (dJ/dmu only)
   // their right-hand sides Ki = f(S_i); lam, the adjoint of the step's end
   // state, becomes the adjoint of its start state
-  std::vector<double> dmu_parts;
+  size_t ndmu = 0;
   // weights of stage k in the adjoint of V': l_k = w_l(k) l + w_a(k) a_(k+1)
   auto wl_of = [&](int k) { return (k == 3 || k == 0) ? dt / 6.0 : dt / 3.0; };
   auto wa_of = [&](int k) { return (k == 2) ? dt : 0.5 * dt; };
@@ -262,11 +263,8 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
                                c->stream));
         AHIP(c, stage(c, ph0, Si[i], tz.p, nullptr, nullptr, nullptr, f0.p, 0.0));
         AHIP(c, axpby_dispatch(f0.p, Ki[i], 1.0 / c->phys.mu, -1.0 / c->phys.mu, n, c->stream));
-        AHIP(c, dot_dispatch(lk.p, f0.p, n, dpart.p, ndot, dsum.p, c->stream));
-        double d = 0.0;
-        AHIP(c, hipMemcpyAsync(&d, dsum.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        AHIP(c, hipStreamSynchronize(c->stream));
-        dmu_parts.push_back(d);
+        AHIP(c, dot_dispatch(lk.p, f0.p, n, dpart.p, ndot, dsum.p + ndmu, c->stream));
+        ++ndmu;
       }
       AHIP(c, adjoint_dispatch(c->p, c->q, mesh_args(c), c->phys, Si[i], Ti[i], c->lam, lk.p,
                                scr.pvol.p, scr.aown.p, scr.anbr.p, acur.p, c->stream));
@@ -401,8 +399,15 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
     }
   }
   double dmu_total = 0.0;
-  for (size_t i = dmu_parts.size(); i-- > 0;)
-    dmu_total += dmu_parts[i];
+  if (ndmu)
+  {
+    std::vector<double> parts(ndmu);
+    AHIP(c, hipMemcpyAsync(parts.data(), dsum.p, ndmu * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    AHIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = ndmu; i-- > 0;)
+      dmu_total += parts[i];
+  }
   // gradient with respect to the initial state in the reference layout
   if (dJdU0_host)
   {
