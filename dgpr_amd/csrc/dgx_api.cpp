@@ -81,7 +81,37 @@ MeshArgs dgx::mesh_args(const dgx_ctx* c)
   m.nelem = c->nelem;
   for (int i = 0; i < 6; ++i)
     m.bounds[i] = c->bounds[i];
+  m.affine = c->affine;
   return m;
+}
+
+// True when q = 1 and every element's bilinear and trilinear node terms
+// vanish to rounding (a parallelepiped, constant Jacobian); the stage
+// kernels then use one Jacobian per element instead of the geometry basis.
+static bool mesh_is_affine(const dgx_mesh* m)
+{
+  if (m->q != 1)
+    return false;
+  const uint64_t ne = m->nx * m->ny * m->nz;
+  for (uint64_t e = 0; e < ne; ++e)
+  {
+    const double* X = m->nodes + e * 24;
+    double twist = 0.0, scale = 0.0;
+    for (int d = 0; d < 3; ++d)
+    {
+      const double x000 = X[d], x100 = X[3 + d], x010 = X[6 + d], x110 = X[9 + d];
+      const double x001 = X[12 + d], x101 = X[15 + d], x011 = X[18 + d], x111 = X[21 + d];
+      twist = std::fmax(twist, std::fabs((x110 - x100) - (x010 - x000)));
+      twist = std::fmax(twist, std::fabs((x101 - x100) - (x001 - x000)));
+      twist = std::fmax(twist, std::fabs((x011 - x010) - (x001 - x000)));
+      twist = std::fmax(twist,
+                        std::fabs(((x111 - x011) - (x101 - x001)) - ((x110 - x010) - (x100 - x000))));
+      scale = std::fmax(scale, std::fmax(std::fabs(x000), std::fabs(x111)));
+    }
+    if (!(twist <= 1e-14 * scale))
+      return false;
+  }
+  return true;
 }
 
 static dgx_status ensure_tables(int device)
@@ -181,6 +211,7 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
   c->device = dev;
   c->p = (int)mesh->p;
   c->q = (int)mesh->q;
+  c->affine = mesh_is_affine(mesh) ? 1 : 0;
   c->nx = (int)mesh->nx;
   c->ny = (int)mesh->ny;
   c->nz = (int)mesh->nz;

@@ -151,6 +151,68 @@ __device__ __forceinline__ void jac_at(const QTab& qt, const double* X, int nd, 
   jacobian<Q>(X, v[0], d[0], v[1], d[1], v[2], d[2], J);
 }
 
+
+// Constant geometry of an affine (parallelepiped) q = 1 element: J = X[e_c]
+// - X[0] everywhere.  The host sets mesh.affine when every element of the
+// mesh is one (bilinear and trilinear node terms zero to rounding); the
+// stage and trace kernels then read J, det J and J^-1 from this per-element
+// record (g[0..8] = J, g[9] = det J, g[10..18] = J^-1) instead of evaluating
+// the geometry basis at every node and face point.  Lanes 0..EPW-1 fill one
+// record each; a barrier must follow before use.
+constexpr int GREC = 19;
+__device__ __forceinline__ void affine_record(const double* __restrict__ nodes, int e, int nelem,
+                                              double* g)
+{
+  if (e >= nelem)
+    return;
+  const double* X = nodes + (size_t)e * 24;
+  double J[3][3], Ji[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    J[d][0] = X[3 + d] - X[d];   // node (1,0,0) - node (0,0,0)
+    J[d][1] = X[6 + d] - X[d];   // node (0,1,0)
+    J[d][2] = X[12 + d] - X[d];  // node (0,0,1)
+  }
+  const double det = det3(J);
+  inv3(J, det, Ji);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+    {
+      g[3 * i + j]      = J[i][j];
+      g[10 + 3 * i + j] = Ji[i][j];
+    }
+  g[9] = det;
+}
+
+// J, J^-1 and det J at a point: from the affine record in the affine
+// instantiation (AFF, q = 1 only), else from the geometry basis.  The two
+// are separate kernels: the general one is unchanged by the affine path.
+template <int Q, bool AFF>
+__device__ __forceinline__ void geometry_at(const QTab& qt, const double* X, const double* g,
+                                            int nd, int e, int i, int j, int k, double J[3][3],
+                                            double Ji[3][3], double& det)
+{
+  if constexpr (AFF)
+  {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+      {
+        J[a][b]  = g[3 * a + b];
+        Ji[a][b] = g[10 + 3 * a + b];
+      }
+    det = g[9];
+    return;
+  }
+  jac_at<Q>(qt, X, nd, e, i, j, k, J);
+  det = det3(J);
+  inv3(J, det, Ji);
+}
+
 // Copies the 1D tables of (P, Q) into LDS: the lane-dependent table reads of
 // the stage kernels then become LDS reads instead of vector loads.
 template <int P, int Q>
@@ -296,11 +358,12 @@ struct FaceDims
 // type: double, or float for the mixed-precision form (the records are
 // computed in double and rounded once on the store).  Called by every
 // thread of the block; sR needs 4*64 doubles.
-template <int P, int Q, typename TT>
+template <int P, int Q, typename TT, bool AFF = false>
 __device__ __forceinline__ void produce_traces(const MeshArgs& mesh, const Phys& ph,
                                                const PTab& pt, const QTab& qt,
                                                const double (*sV)[5][FaceDims<P>::NB],
                                                const double (*sX)[(Q + 1) * (Q + 1) * (Q + 1) * 3],
+                                               const double (*sG)[GREC],
                                                const int (*sNb)[6], const int (*sBt)[6],
                                                double* sR, int e0, TT* __restrict__ Tout,
                                                int* status)
@@ -351,9 +414,8 @@ __device__ __forceinline__ void produce_traces(const MeshArgs& mesh, const Phys&
       const int gi0 = (nd == 0) ? 0 : a;
       const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
       const int gi2 = (nd == 2) ? 0 : b;
-      double J[3][3], Ji[3][3];
-      jac_at<Q>(qt, sX[l], nd, hi, gi0, gi1, gi2, J);
-      inv3(J, det3(J), Ji);
+      double J[3][3], Ji[3][3], det;
+      geometry_at<Q, AFF>(qt, sX[l], sG[l], nd, hi, gi0, gi1, gi2, J, Ji, det);
       double nv[3];
       face_normal(J, nd, hi, nv);
       double gr[3][4];
@@ -400,7 +462,7 @@ __device__ __forceinline__ void produce_traces(const MeshArgs& mesh, const Phys&
 
 // Trace records of a Gauss-nodal state V (the first stage after the state
 // was set from outside, and stand-alone residual evaluations).
-template <int P, int Q, typename TT>
+template <int P, int Q, typename TT, bool AFF>
 __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
                                                        const double* __restrict__ V,
                                                        TT* __restrict__ Tout, int* status)
@@ -410,6 +472,7 @@ __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
   constexpr int NG = (Q + 1) * (Q + 1) * (Q + 1);
   __shared__ double sV[EPW][5][NB];
   __shared__ double sX[EPW][NG * 3];
+  __shared__ double sG[EPW][GREC];
   __shared__ double sR[4 * 64];
   __shared__ PTab sPT;
   __shared__ QTab sQT;
@@ -432,8 +495,10 @@ __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
     sX[x / (NG * 3)][x % (NG * 3)] =
         (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
   }
+  if (AFF && tid < EPW)
+    affine_record(mesh.nodes, e0 + tid, mesh.nelem, sG[tid]);
   __syncthreads();
-  produce_traces<P, Q, TT>(mesh, ph, sPT, sQT, sV, sX, sNb, sBt, sR, e0, Tout, status);
+  produce_traces<P, Q, TT, AFF>(mesh, ph, sPT, sQT, sV, sX, sG, sNb, sBt, sR, e0, Tout, status);
 }
 
 // One full right-hand-side evaluation f_g = B3 f of EPW elements fused with
@@ -442,7 +507,7 @@ __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
 // holds the BR2 lifting coefficient of element e's side of its face f.
 // Body of the stage kernel for block `block` of one mesh: shared by the
 // single-mesh kernel and the batched ensemble kernel.
-template <int P, int Q, typename TT>
+template <int P, int Q, typename TT, bool AFF = false>
 __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
                                            const StageArgs& sa, const double* __restrict__ lam,
                                            int* status, int block)
@@ -453,6 +518,7 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
 
   __shared__ double sV[EPW][5][NB];
   __shared__ double sX[EPW][NG * 3];      // own geometry nodes
+  __shared__ double sG[EPW][GREC];        // affine-element geometry records
   __shared__ double sR[FD::RG];  // volume flux of one direction, then one face pass
   __shared__ PTab sPT;
   __shared__ QTab sQT;
@@ -483,6 +549,8 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
     int ee = e0 + l;
     sX[l][x % (NG * 3)] = (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
   }
+  if (AFF && tid < EPW)
+    affine_record(mesh.nodes, e0 + tid, mesh.nelem, sG[tid]);
   __syncthreads();
 
   const int idx[3] = {nd0 % N, (nd0 / N) % N, nd0 / (N * N)};
@@ -494,10 +562,8 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
   // ---- volume: velocity gradient, fluxes ----
   if (vact)
   {
-    double J[3][3];
-    jac_at<Q>(qt, sX[le], -1, 0, idx[0], idx[1], idx[2], J);
-    double det = det3(J);
-    inv3(J, det, Ji);
+    double J[3][3], det;
+    geometry_at<Q, AFF>(qt, sX[le], sG[le], -1, 0, idx[0], idx[1], idx[2], J, Ji, det);
     wJ = pt.w[idx[0]] * pt.w[idx[1]] * pt.w[idx[2]] * det;
     double Vq[5], gr[3][4];
 #pragma unroll
@@ -588,9 +654,8 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
       const int gi0 = (nd == 0) ? 0 : a;
       const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
       const int gi2 = (nd == 2) ? 0 : b;
-      double J[3][3], JiL[3][3];
-      jac_at<Q>(qt, sX[l], nd, hi, gi0, gi1, gi2, J);
-      inv3(J, det3(J), JiL);
+      double J[3][3], JiL[3][3], detL;
+      geometry_at<Q, AFF>(qt, sX[l], sG[l], nd, hi, gi0, gi1, gi2, J, JiL, detL);
       double nv[3];
       const double fJ = face_normal(J, nd, hi, nv);
       const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
@@ -736,20 +801,20 @@ __device__ __forceinline__ void stage_body(const MeshArgs& mesh, const Phys& ph,
   if (sa.Tout)
   {
     __syncthreads();
-    produce_traces<P, Q, TT>(mesh, ph, pt, qt, sV, sX, sNb, sBt, sR, e0,
-                             reinterpret_cast<TT*>(sa.Tout), status);
+    produce_traces<P, Q, TT, AFF>(mesh, ph, pt, qt, sV, sX, sG, sNb, sBt, sR, e0,
+                                  reinterpret_cast<TT*>(sa.Tout), status);
   }
 }
 
 // W is the occupancy target (waves per SIMD) handed to the register
 // allocator; the launch picks one of the compiled targets per (P, Q).  TT is
 // the type of the trace records read and written (float: mixed precision).
-template <int P, int Q, int W, typename TT>
+template <int P, int Q, int W, typename TT, bool AFF>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
                           int* status)
 {
-  stage_body<P, Q, TT>(mesh, ph, sa, lam, status, blockIdx.x);
+  stage_body<P, Q, TT, AFF>(mesh, ph, sa, lam, status, blockIdx.x);
 }
 
 // Batched form: the blocks of several independent meshes of the same (P, Q)
@@ -1168,24 +1233,35 @@ __global__ __launch_bounds__(256) void dgx_mean_kernel(const double* __restrict_
 
 // ---- launch helpers (host) -----------------------------------------------
 
-template <int P, int Q, typename TT>
+template <int P, int Q, typename TT, bool AFF>
 hipError_t launch_rhs_t(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
                         int* status, int waves, hipStream_t st)
 {
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
   if (waves == 1)
-    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 1, TT>), dim3(nblk), dim3(64), 0, st, m, ph,
-                       sa, lam, status);
+    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 1, TT, AFF>), dim3(nblk), dim3(64), 0, st, m,
+                       ph, sa, lam, status);
   else if (waves == 2)
-    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 2, TT>), dim3(nblk), dim3(64), 0, st, m, ph,
-                       sa, lam, status);
+    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 2, TT, AFF>), dim3(nblk), dim3(64), 0, st, m,
+                       ph, sa, lam, status);
   else if (waves == 3)
-    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 3, TT>), dim3(nblk), dim3(64), 0, st, m, ph,
-                       sa, lam, status);
+    hipLaunchKernelGGL((dgx_rhs_stage_kernel<P, Q, 3, TT, AFF>), dim3(nblk), dim3(64), 0, st, m,
+                       ph, sa, lam, status);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
+}
+
+// affine q = 1 meshes take the constant-Jacobian instantiation
+template <int P, int Q, typename TT>
+hipError_t launch_rhs_a(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
+                        int* status, int waves, hipStream_t st)
+{
+  if constexpr (Q == 1)
+    if (m.affine)
+      return launch_rhs_t<P, Q, TT, true>(m, ph, sa, lam, status, waves, st);
+  return launch_rhs_t<P, Q, TT, false>(m, ph, sa, lam, status, waves, st);
 }
 
 template <int P, int Q>
@@ -1193,8 +1269,8 @@ hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, co
                       int* status, int waves, hipStream_t st)
 {
   if (sa.trace32)
-    return launch_rhs_t<P, Q, float>(m, ph, sa, lam, status, waves, st);
-  return launch_rhs_t<P, Q, double>(m, ph, sa, lam, status, waves, st);
+    return launch_rhs_a<P, Q, float>(m, ph, sa, lam, status, waves, st);
+  return launch_rhs_a<P, Q, double>(m, ph, sa, lam, status, waves, st);
 }
 
 template <int P, int Q>
@@ -1221,12 +1297,19 @@ hipError_t launch_trace(const MeshArgs& m, const Phys& ph, const double* V, doub
 {
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
-  if (trace32)
-    hipLaunchKernelGGL((dgx_trace_kernel<P, Q, float>), dim3(nblk), dim3(64), 0, st, m, ph, V,
-                       reinterpret_cast<float*>(Tout), status);
+  const bool aff = (Q == 1) && m.affine;
+  if (trace32 && aff)
+    hipLaunchKernelGGL((dgx_trace_kernel<P, Q, float, Q == 1>), dim3(nblk), dim3(64), 0, st, m,
+                       ph, V, reinterpret_cast<float*>(Tout), status);
+  else if (trace32)
+    hipLaunchKernelGGL((dgx_trace_kernel<P, Q, float, false>), dim3(nblk), dim3(64), 0, st, m,
+                       ph, V, reinterpret_cast<float*>(Tout), status);
+  else if (aff)
+    hipLaunchKernelGGL((dgx_trace_kernel<P, Q, double, Q == 1>), dim3(nblk), dim3(64), 0, st, m,
+                       ph, V, Tout, status);
   else
-    hipLaunchKernelGGL((dgx_trace_kernel<P, Q, double>), dim3(nblk), dim3(64), 0, st, m, ph, V,
-                       Tout, status);
+    hipLaunchKernelGGL((dgx_trace_kernel<P, Q, double, false>), dim3(nblk), dim3(64), 0, st, m,
+                       ph, V, Tout, status);
   return hipGetLastError();
 }
 

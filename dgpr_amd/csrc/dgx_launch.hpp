@@ -50,6 +50,7 @@ struct MeshArgs
   int nx, ny, nz;
   int nelem;
   int bounds[6];
+  int affine;  // every element is a q = 1 parallelepiped (constant Jacobian)
 };
 
 // Epilogue of one right-hand-side evaluation:
