@@ -21,7 +21,7 @@
 // PART 0: the volume terms (into pvol); PART 1: the face terms (into aown,
 // anbr).  Separate launches: the volume part needs about half the registers
 // of the face part and runs at a higher occupancy on its own.
-template <int P, int Q, int PART>
+template <int P, int Q, int PART, bool AFF>
 __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph,
                                                           const double* __restrict__ V,
                                                           const double* __restrict__ Tin,
@@ -37,6 +37,7 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
   __shared__ double sV[EPW][5][NB];
   __shared__ double sA[EPW][5][NB];  // adjoint of acc
   __shared__ double sX[EPW][NG * 3];
+  __shared__ double sG[EPW][GREC];  // affine-element geometry records
   __shared__ double sR[(4 * NB * EPW > 4 * 64) ? 4 * NB * EPW : 4 * 64];
   __shared__ PTab sPT;
   __shared__ QTab sQT;
@@ -55,6 +56,8 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
     sX[x / (NG * 3)][x % (NG * 3)] =
         (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
   }
+  if (AFF && tid < EPW)
+    affine_record(mesh.nodes, e0 + tid, mesh.nelem, sG[tid]);
   const int idx[3] = {nd0 % N, (nd0 / N) % N, nd0 / (N * N)};
   double Ji[3][3], wJ = 1.0;
   if (PART == 0 && vact)
@@ -66,10 +69,8 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
   __syncthreads();
   if (vact)
   {
-    double J[3][3];
-    jac_at<Q>(qt, sX[le], -1, 0, idx[0], idx[1], idx[2], J);
-    double det = det3(J);
-    inv3(J, det, Ji);
+    double J[3][3], det;
+    geometry_at<Q, AFF>(qt, sX[le], sG[le], -1, 0, idx[0], idx[1], idx[2], J, Ji, det);
     wJ = pt.w[idx[0]] * pt.w[idx[1]] * pt.w[idx[2]] * det;
     const double iwJ = rcp(wJ);
 #pragma unroll
@@ -251,9 +252,8 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
       const int gi0 = (nd == 0) ? 0 : a;
       const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
       const int gi2 = (nd == 2) ? 0 : b;
-      double J[3][3], JiL[3][3];
-      jac_at<Q>(qt, sX[l], nd, hi, gi0, gi1, gi2, J);
-      inv3(J, det3(J), JiL);
+      double J[3][3], JiL[3][3], detL;
+      geometry_at<Q, AFF>(qt, sX[l], sG[l], nd, hi, gi0, gi1, gi2, J, JiL, detL);
       double nv[3];
       const double fJ = face_normal(J, nd, hi, nv);
       const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
   }
 }
 
-template <int P, int Q>
+template <int P, int Q, bool AFF>
 __global__ __launch_bounds__(64) void dgx_adj_trace_kernel(MeshArgs mesh, Phys ph,
                                                            const double* __restrict__ V,
                                                            const double* __restrict__ aown,
@@ -341,6 +341,7 @@ __global__ __launch_bounds__(64) void dgx_adj_trace_kernel(MeshArgs mesh, Phys p
   constexpr int NG = (Q + 1) * (Q + 1) * (Q + 1);
   __shared__ double sV[EPW][5][NB];
   __shared__ double sX[EPW][NG * 3];
+  __shared__ double sG[EPW][GREC];  // affine-element geometry records
   __shared__ double sU[4 * 64];  // trace values (rho, m) of the pass
   __shared__ double sQ[8 * 64];  // adjoint tangential derivatives
   __shared__ double sT[9 * 64];  // adjoint trace values and normal derivatives
@@ -358,6 +359,8 @@ __global__ __launch_bounds__(64) void dgx_adj_trace_kernel(MeshArgs mesh, Phys p
     sX[x / (NG * 3)][x % (NG * 3)] =
         (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
   }
+  if (AFF && tid < EPW)
+    affine_record(mesh.nodes, e0 + tid, mesh.nelem, sG[tid]);
   double pv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (vact)
   {
@@ -410,9 +413,8 @@ __global__ __launch_bounds__(64) void dgx_adj_trace_kernel(MeshArgs mesh, Phys p
       const int gi0 = (nd == 0) ? 0 : a;
       const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
       const int gi2 = (nd == 2) ? 0 : b;
-      double J[3][3], Ji[3][3];
-      jac_at<Q>(qt, sX[l], nd, hi, gi0, gi1, gi2, J);
-      inv3(J, det3(J), Ji);
+      double J[3][3], Ji[3][3], det;
+      geometry_at<Q, AFF>(qt, sX[l], sG[l], nd, hi, gi0, gi1, gi2, J, Ji, det);
       double nv[3];
       face_normal(J, nd, hi, nv);
       double gr[3][4];

@@ -27,6 +27,7 @@ namespace
 struct EnsGroup
 {
   int p, q;
+  int affine;  // every member an affine q = 1 mesh
   std::vector<int> members;
   int nblocks;
   int waves;
@@ -133,7 +134,7 @@ static hipError_t enqueue_stage(dgx_ensemble* en, int s, int only_group = -1)
       continue;
     EnsGroup& g = en->groups[gi];
     hipError_t e = rhs_batch_dispatch(g.p, g.q, g.items[s], (int)g.members.size(), g.nblocks,
-                                      en->status, g.waves, en->stream);
+                                      en->status, g.waves, g.affine != 0, en->stream);
     if (e != hipSuccess)
       return e;
   }
@@ -195,13 +196,14 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
     dgx_ctx* c = en->ctx[i];
     EnsGroup* g = nullptr;
     for (EnsGroup& x : en->groups)
-      if (x.p == c->p && x.q == c->q)
+      if (x.p == c->p && x.q == c->q && x.affine == c->affine)
         g = &x;
     if (!g)
     {
       EnsGroup ng;
       ng.p = c->p;
       ng.q = c->q;
+      ng.affine = c->affine;
       ng.nblocks = 0;
       ng.waves = default_stage_waves(c->p, c->q);
       for (int s = 0; s < 4; ++s)

@@ -820,7 +820,7 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
 // Batched form: the blocks of several independent meshes of the same (P, Q)
 // in one launch.  items[] is sorted by first block; each block finds its
 // member by a uniform binary search.
-template <int P, int Q, int W>
+template <int P, int Q, int W, bool AFF>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 void dgx_rhs_batch_kernel(const BatchItem* __restrict__ items, int nitems, int* status)
 {
@@ -835,7 +835,7 @@ void dgx_rhs_batch_kernel(const BatchItem* __restrict__ items, int nitems, int* 
       hi = mid - 1;
   }
   const BatchItem& it = items[lo];
-  stage_body<P, Q, double>(it.mesh, it.ph, it.sa, it.lam, status, b - it.block0);
+  stage_body<P, Q, double, AFF>(it.mesh, it.ph, it.sa, it.lam, status, b - it.block0);
 }
 
 // BR2 lifting coefficient of each element's side of each of its faces:
@@ -1273,22 +1273,34 @@ hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, co
   return launch_rhs_a<P, Q, double>(m, ph, sa, lam, status, waves, st);
 }
 
-template <int P, int Q>
-hipError_t launch_rhs_batch(const BatchItem* items, int nitems, int nblk, int* status, int waves,
-                            hipStream_t st)
+template <int P, int Q, bool AFF>
+hipError_t launch_rhs_batch_t(const BatchItem* items, int nitems, int nblk, int* status,
+                              int waves, hipStream_t st)
 {
   if (waves == 1)
-    hipLaunchKernelGGL((dgx_rhs_batch_kernel<P, Q, 1>), dim3(nblk), dim3(64), 0, st, items,
+    hipLaunchKernelGGL((dgx_rhs_batch_kernel<P, Q, 1, AFF>), dim3(nblk), dim3(64), 0, st, items,
                        nitems, status);
   else if (waves == 2)
-    hipLaunchKernelGGL((dgx_rhs_batch_kernel<P, Q, 2>), dim3(nblk), dim3(64), 0, st, items,
+    hipLaunchKernelGGL((dgx_rhs_batch_kernel<P, Q, 2, AFF>), dim3(nblk), dim3(64), 0, st, items,
                        nitems, status);
   else if (waves == 3)
-    hipLaunchKernelGGL((dgx_rhs_batch_kernel<P, Q, 3>), dim3(nblk), dim3(64), 0, st, items,
+    hipLaunchKernelGGL((dgx_rhs_batch_kernel<P, Q, 3, AFF>), dim3(nblk), dim3(64), 0, st, items,
                        nitems, status);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
+}
+
+// groups whose members are all affine q = 1 meshes take the constant-
+// Jacobian instantiation
+template <int P, int Q>
+hipError_t launch_rhs_batch(const BatchItem* items, int nitems, int nblk, int* status, int waves,
+                            bool affine, hipStream_t st)
+{
+  if constexpr (Q == 1)
+    if (affine)
+      return launch_rhs_batch_t<P, Q, true>(items, nitems, nblk, status, waves, st);
+  return launch_rhs_batch_t<P, Q, false>(items, nitems, nblk, status, waves, st);
 }
 
 template <int P, int Q>
@@ -1379,9 +1391,9 @@ hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const S
 }
 
 hipError_t rhs_batch_dispatch(int p, int q, const BatchItem* items_dev, int nitems, int nblocks,
-                              int* status, int waves, hipStream_t st)
+                              int* status, int waves, bool affine, hipStream_t st)
 {
-#define CALL_RB(P, Q) launch_rhs_batch<P, Q>(items_dev, nitems, nblocks, status, waves, st)
+#define CALL_RB(P, Q) launch_rhs_batch<P, Q>(items_dev, nitems, nblocks, status, waves, affine, st)
   DGX_DISPATCH_PQ(p, q, CALL_RB)
 #undef CALL_RB
 }
@@ -1460,26 +1472,38 @@ hipError_t mean_dispatch(const double* U, double* mean, size_t n, double old_ran
   return hipGetLastError();
 }
 
+template <int P, int Q, bool AFF>
+hipError_t launch_adjoint_t(const MeshArgs& m, const Phys& ph, const double* V, const double* Tin,
+                            const double* lam, const double* psi, double* pvol, double* aown,
+                            double* anbr, double* psiV, hipStream_t st)
+{
+  constexpr int EPW = Dims<P, Q>::EPW;
+  int nblk = (m.nelem + EPW - 1) / EPW;
+  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q, 0, AFF>), dim3(nblk), dim3(64), 0, st, m, ph, V,
+                     Tin, lam, psi, pvol, aown, anbr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q, 1, AFF>), dim3(nblk), dim3(64), 0, st, m, ph, V,
+                     Tin, lam, psi, pvol, aown, anbr);
+  e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  hipLaunchKernelGGL((dgx_adj_trace_kernel<P, Q, AFF>), dim3(nblk), dim3(64), 0, st, m, ph, V,
+                     aown, anbr, pvol, psiV);
+  return hipGetLastError();
+}
+
+// affine q = 1 meshes take the constant-Jacobian instantiation
 template <int P, int Q>
 hipError_t launch_adjoint(const MeshArgs& m, const Phys& ph, const double* V, const double* Tin,
                           const double* lam, const double* psi, double* pvol, double* aown,
                           double* anbr, double* psiV, hipStream_t st)
 {
-  constexpr int EPW = Dims<P, Q>::EPW;
-  int nblk = (m.nelem + EPW - 1) / EPW;
-  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q, 0>), dim3(nblk), dim3(64), 0, st, m, ph, V, Tin,
-                     lam, psi, pvol, aown, anbr);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess)
-    return e;
-  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q, 1>), dim3(nblk), dim3(64), 0, st, m, ph, V, Tin,
-                     lam, psi, pvol, aown, anbr);
-  e = hipGetLastError();
-  if (e != hipSuccess)
-    return e;
-  hipLaunchKernelGGL((dgx_adj_trace_kernel<P, Q>), dim3(nblk), dim3(64), 0, st, m, ph, V, aown,
-                     anbr, pvol, psiV);
-  return hipGetLastError();
+  if constexpr (Q == 1)
+    if (m.affine)
+      return launch_adjoint_t<P, Q, true>(m, ph, V, Tin, lam, psi, pvol, aown, anbr, psiV, st);
+  return launch_adjoint_t<P, Q, false>(m, ph, V, Tin, lam, psi, pvol, aown, anbr, psiV, st);
 }
 
 hipError_t adjoint_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,

@@ -93,9 +93,10 @@ struct BatchItem
 hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
                         const double* lam, int* status, int waves, hipStream_t st);
 // one stage of several meshes of the same (p, q): items_dev (device memory,
-// sorted by block0) covers nblocks blocks in total
+// sorted by block0) covers nblocks blocks in total; affine: every member is
+// an affine q = 1 mesh (MeshArgs::affine)
 hipError_t rhs_batch_dispatch(int p, int q, const BatchItem* items_dev, int nitems, int nblocks,
-                              int* status, int waves, hipStream_t st);
+                              int* status, int waves, bool affine, hipStream_t st);
 // grid blocks of one mesh in a stage launch
 int stage_blocks(int p, int nelem);
 // face trace records Tout of the Gauss-nodal state V (and the boundary

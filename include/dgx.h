@@ -202,9 +202,10 @@ dgx_status dgx_get_mean(dgx_ctx* ctx, double* mean_host);
  * An ensemble owns one context per member (meshes of any size, p and q; own
  * parameters).  dgx_ensemble_step advances every member by nsteps steps of
  * its own dt: per RK stage, the members sharing (p, q) run as one launch of
- * the batched stage kernel; with use_graph != 0 one step of the whole
- * ensemble is captured once in a hipGraph and replayed.  Results equal those
- * of dgx_step on each member.  The call only enqueues work on the ensemble's
+ * the batched stage kernel (affine q = 1 meshes, whose kernels use one
+ * Jacobian per element, form their own groups); with use_graph != 0 one step
+ * of the whole ensemble is captured once in a hipGraph and replayed.  Results
+ * equal those of dgx_step on each member.  The call only enqueues work on the ensemble's
  * stream: call dgx_ensemble_sync before using a member context again. */
 typedef struct dgx_ensemble dgx_ensemble;
 dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_params* params,
