@@ -1479,7 +1479,9 @@ hipError_t launch_adjoint_t(const MeshArgs& m, const Phys& ph, const double* V, 
 {
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
-  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q, 0, AFF>), dim3(nblk), dim3(64), 0, st, m, ph, V,
+  // the volume part keeps the general geometry: its affine form holds 20
+  // more VGPRs (146 vs 126 at p = 2), one wave/SIMD less, and runs slower
+  hipLaunchKernelGGL((dgx_adj_main_kernel<P, Q, 0, false>), dim3(nblk), dim3(64), 0, st, m, ph, V,
                      Tin, lam, psi, pvol, aown, anbr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
