@@ -90,7 +90,9 @@ typedef struct dgx_ctx dgx_ctx;
  * (solve_cuda.cu:171) together with cuda_residual_workspace(...)
  * (solve_cuda.cu:303): uploads the mesh, allocates the state and the stage
  * buffers of both RK schemes on the current HIP device and creates one
- * stream.  Supported: 1 <= p <= 3, 1 <= q <= 3. */
+ * stream.  Supported: 1 <= p <= 3, 1 <= q <= 3.  A q = 1 mesh whose elements
+ * are all parallelepipeds runs the constant-Jacobian kernels (chosen here,
+ * nothing to set). */
 dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params,
                       dgx_ctx** out);
 
