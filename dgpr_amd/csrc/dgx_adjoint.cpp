@@ -224,24 +224,43 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   // (and f for dJ/dmu) from the forward sweep when that fits in three
   // quarters of the free device memory, so the reverse sweep recomputes
   // nothing; otherwise (or for interval >= 1) checkpoint every K steps.
+  // The store stays with the context for later calls (an optimisation loop
+  // calls this repeatedly; freeing tens of GB has taken seconds) until
+  // dgx_adjoint_release or dgx_destroy, and is replaced only when too small.
   const size_t per_step = 4 * (n + ntr) + (dJdmu ? 4 * n : 0);
-  DevBuf store;
-  bool store_all = false;
+  double* store = nullptr;
   if (interval == 0)
   {
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) == hipSuccess &&
-        (double)nsteps * per_step * sizeof(double) <= 0.75 * (double)fr)
+    const size_t need = (size_t)nsteps * per_step * sizeof(double);
+    if (c->adj_store && c->adj_store_bytes >= need)
+      store = c->adj_store;
+    else
     {
-      store_all = alloc(store, nsteps * per_step) == hipSuccess;
-      if (!store_all)
-        (void)hipGetLastError();  // fall back to checkpointing
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess &&
+          (double)need <= 0.75 * ((double)fr + (double)c->adj_store_bytes))
+      {
+        AHIP(c, hipStreamSynchronize(c->stream));
+        if (c->adj_store)
+          AHIP(c, hipFree(c->adj_store));
+        c->adj_store = nullptr;
+        c->adj_store_bytes = 0;
+        double* p = nullptr;
+        if (hipMalloc(&p, need) == hipSuccess)
+        {
+          c->adj_store = store = p;
+          c->adj_store_bytes = need;
+        }
+        else
+          (void)hipGetLastError();  // fall back to checkpointing
+      }
     }
   }
+  const bool store_all = store != nullptr;
   const uint64_t K = interval ? interval : (uint64_t)std::ceil(std::sqrt((double)nsteps));
-  auto Sp = [&](uint64_t s, int i) { return store.p + (s * 4 + i) * (n + ntr); };
+  auto Sp = [&](uint64_t s, int i) { return store + (s * 4 + i) * (n + ntr); };
   auto Tp = [&](uint64_t s, int i) { return Sp(s, i) + n; };
-  auto Kp = [&](uint64_t s, int i) { return store.p + nsteps * 4 * (n + ntr) + (s * 4 + i) * n; };
+  auto Kp = [&](uint64_t s, int i) { return store + nsteps * 4 * (n + ntr) + (s * 4 + i) * n; };
 
   // reverse of one step: stage states Si with records Ti and (dJ/dmu only)
   // their right-hand sides Ki = f(S_i); lam, the adjoint of the step's end
@@ -430,6 +449,20 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
     *J = Jv;
   if (dJdmu)
     *dJdmu = dmu_total;
+  return DGX_OK;
+}
+
+dgx_status dgx_adjoint_release(dgx_ctx* c)
+{
+  if (!c)
+    return DGX_ERR_INVALID;
+  if (c->adj_store)
+  {
+    AHIP(c, hipStreamSynchronize(c->stream));
+    AHIP(c, hipFree(c->adj_store));
+    c->adj_store = nullptr;
+    c->adj_store_bytes = 0;
+  }
   return DGX_OK;
 }
 

@@ -284,6 +284,8 @@ dgx_status dgx_destroy(dgx_ctx* c)
   for (double* b : bufs)
     if (b)
       (void)hipFree(b);
+  if (c->adj_store)
+    (void)hipFree(c->adj_store);
   if (c->status)
     (void)hipFree(c->status);
   if (c->stream)

@@ -36,6 +36,8 @@ struct dgx_ctx
   int prec;         // DGX_PREC_FP64 or DGX_PREC_MIXED (float trace records)
   int affine;       // q = 1 and every element a parallelepiped
   double* mean;     // running mean (Gauss-nodal), allocated on first use
+  double* adj_store;       // stored stages of the last dgx_adjoint_rk4 (kept:
+  size_t adj_store_bytes;  // freeing tens of GB can take seconds)
   int* status;      // kernel-side error flags
   int npartial;
   size_t bytes;

@@ -115,6 +115,7 @@ def lib():
         L.dgx_kinetic_energy.argtypes = [P_ctx, PD]
         L.dgx_residual_vjp.argtypes = [P_ctx, PD, PD]
         L.dgx_adjoint_rk4.argtypes = [P_ctx, D, U64, U64, PD, PD, PD]
+        L.dgx_adjoint_release.argtypes = [P_ctx]
         L.dgx_ensemble_create.argtypes = [C.c_int, C.POINTER(_CMesh), C.POINTER(Params),
                                           C.POINTER(C.c_void_p)]
         L.dgx_ensemble_destroy.argtypes = [C.c_void_p]
@@ -147,7 +148,7 @@ def lib():
                    "dgx_ensemble_create", "dgx_ensemble_destroy", "dgx_ensemble_member",
                    "dgx_ensemble_step", "dgx_ensemble_sync", "dgx_ensemble_get_stream",
                    "dgx_ensemble_autotune", "dgx_kinetic_energy", "dgx_residual_vjp",
-                   "dgx_adjoint_rk4"):
+                   "dgx_adjoint_rk4", "dgx_adjoint_release"):
             getattr(L, nm).restype = st
         _lib = L
     return _lib
@@ -400,6 +401,10 @@ class Solver:
         _chk(lib().dgx_adjoint_rk4(self.ctx, dt, int(nsteps), int(interval), C.byref(J), _p(g),
                                    C.byref(dmu) if want_mu else None), self.ctx)
         return J.value, g, (dmu.value if want_mu else None)
+
+    def adjoint_release(self):
+        """Free the stored-stage buffer adjoint_rk4 keeps between calls."""
+        _chk(lib().dgx_adjoint_release(self.ctx), self.ctx)
 
     def sample_output(self, var, N):
         """eval_output (render/vrtxgen.cpp:25-59) of the current state on the

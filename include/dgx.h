@@ -251,9 +251,17 @@ dgx_status dgx_residual_vjp(dgx_ctx* ctx, const double* w_host, double* g_host);
  * when nsteps * 4 * (state + records [+ state]) fits in three quarters of
  * the free device memory, so nothing is recomputed; otherwise checkpoints
  * every ceil(sqrt(nsteps)) steps.  Both schedules give the same gradients
- * bit for bit. */
+ * bit for bit.  The stored stages stay allocated on the context for later
+ * calls (a buffer that is large enough is reused; freeing tens of GB can
+ * take seconds) until dgx_adjoint_release or dgx_destroy; the free-memory
+ * test counts the held buffer as free. */
 dgx_status dgx_adjoint_rk4(dgx_ctx* ctx, double dt, uint64_t nsteps, uint64_t interval,
                            double* J, double* dJdU0_host, double* dJdmu);
+
+/* Frees the stored-stage buffer that dgx_adjoint_rk4 keeps on the context
+ * (no counterpart in the reference, which has no adjoint).  No-op when none
+ * is held. */
+dgx_status dgx_adjoint_release(dgx_ctx* ctx);
 
 /* ---- precision form (no counterpart in the reference: its CPU solver is
  * FP64 and its CUDA driver FP32 throughout, cudg.cu:26-27) ----------------- */

@@ -120,3 +120,32 @@ def test_adjoint_needs_periodic_mesh(built):
         with pytest.raises(dgx.DgxError) as ei:
             s.residual_vjp(np.ones(c.mesh.state_size))
         assert ei.value.status == dgx.ERR_UNSUPPORTED
+
+
+def test_stored_stages_kept_between_calls(built):
+    """The stored-stage buffer stays on the context: a later call that fits
+    reuses it, a larger one replaces it, dgx_adjoint_release frees it (twice
+    is a no-op).  Every call agrees bit for bit with the same call on a fresh
+    context."""
+    c = small_cases()["tgv2_p2_curved"]
+    calls = [(5, True), (3, False), (8, True), "release", (5, True), "release", "release",
+             (2, False)]
+
+    def fresh(nsteps, want_mu):
+        with dgx.Solver(c.mesh, c.params) as f:
+            f.set_state(c.U0)
+            J, g, dmu = f.adjoint_rk4(c.dt, nsteps, want_mu=want_mu)
+            return J, g, dmu, f.get_state()
+
+    with dgx.Solver(c.mesh, c.params) as s:
+        for call in calls:
+            if call == "release":
+                s.adjoint_release()
+                continue
+            nsteps, want_mu = call
+            print("call", nsteps, "steps, dJ/dmu", want_mu, flush=True)
+            s.set_state(c.U0)
+            J, g, dmu = s.adjoint_rk4(c.dt, nsteps, want_mu=want_mu)
+            Jf, gf, dmuf, Uf = fresh(nsteps, want_mu)
+            assert J == Jf and dmu == dmuf
+            assert np.array_equal(g, gf) and np.array_equal(s.get_state(), Uf)

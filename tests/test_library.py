@@ -171,6 +171,7 @@ def test_entry_points_reject_null_arguments():
     assert L.dgx_kinetic_energy(None, C.byref(J)) == dgx.ERR_INVALID
     assert L.dgx_residual_vjp(None, buf, buf) == dgx.ERR_INVALID
     assert L.dgx_adjoint_rk4(None, 1e-3, 2, 0, C.byref(J), buf, None) == dgx.ERR_INVALID
+    assert L.dgx_adjoint_release(None) == dgx.ERR_INVALID
     assert L.dgx_turbulent_state(None, None, 1.0, buf) == dgx.ERR_INVALID
     c = cases.tgv_case(n=2, p=1)
     cm = c.mesh.cstruct()
