@@ -6,7 +6,11 @@
 ``KineticEnergyRK4``  J(U0, mu) = volume-integrated kinetic energy after n
     RK4 steps (stage algebra of time_distributed.cpp:46-102); backward gives
     dJ/dU0 and dJ/dmu through ``dgx_adjoint_rk4`` (discrete adjoint of the
-    stages with sqrt(n) step checkpointing and recompute).
+    stages: every stage state kept from the forward sweep when it fits in
+    device memory, else checkpoints every ceil(sqrt(n)) steps, or every
+    ``interval`` steps when given).  The stored stages stay with the solver
+    for later backward passes until ``solver.adjoint_release()`` or
+    ``solver.close()``.
 
 Both act on periodic meshes (the adjoint kernels cover interior faces
 only).  States are float64 tensors in the simstate layout U[e][r][bf]:
