@@ -1,7 +1,9 @@
 """Batched ensembles: members of different size, p, q, boundary conditions
-and parameters advanced together (one batched launch per (p, q) group and
-RK stage, optionally replayed from a captured hipGraph) must reproduce the
-single-case stepper on each member, and the oracle."""
+and parameters advanced together (one batched launch per (p, q, affine)
+group and RK stage, optionally replayed from a captured hipGraph) must
+reproduce the single-case stepper on each member, and the oracle.  The
+members include affine and curved q = 1 meshes of the same p = 2, which
+form two groups (constant-Jacobian and general kernels)."""
 import numpy as np
 import pytest
 
@@ -22,6 +24,7 @@ def members():
         cases.tgv_case(nxyz=(4, 3, 3), p=2, seed=44, amp=0.2, M=0.15),
         cases.channel_case(3, 3, 2, p=2, q=2, seed=45),
         cases.tgv_case(n=2, p=3, seed=46, M=0.3, Re=5000.0),
+        cases.tgv_case(n=3, p=2, seed=47, M=0.2),
     ]
 
 
