@@ -69,6 +69,11 @@ CASES = {
     "tgv3_p3": lambda: cases.tgv_case(n=3, p=3, seed=13, nsteps=5),
     "tgv_curved_p3": lambda: cases.tgv_case(nxyz=(3, 4, 2), p=3, seed=14, amp=0.15, nsteps=3),
     "tgv_curved_p2": lambda: cases.tgv_case(nxyz=(4, 3, 3), p=2, seed=24, amp=0.2, nsteps=3),
+    # nodes moved by up to 1.7e-8: no longer affine (the classifier's bar is
+    # 1e-14 of the element size), so the general kernels run; the shift moves
+    # the oracle's f by 1e-7, so constant-Jacobian kernels, which would miss
+    # the element twist, could not meet the 1e-12 bar
+    "tgv_near_affine_p3": lambda: cases.tgv_case(n=3, p=3, seed=15, amp=1e-8, nsteps=3),
     "channel_p1": lambda: cases.channel_case(4, 4, 3, p=1, seed=15, nsteps=5),
     "channel_p2": lambda: cases.channel_case(3, 4, 2, p=2, seed=16, nsteps=4),
     "channel_p3": lambda: cases.channel_case(3, 3, 2, p=3, seed=17, nsteps=3),
