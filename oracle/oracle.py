@@ -13,11 +13,16 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# the same C source built with -DORC_EXTENDED: x87 extended precision
+LIBX_PATH = os.path.join(HERE, "liboracle_x.so")
 
 _lib = None
+_libx = None
 
 D = C.c_double
 PD = C.POINTER(C.c_double)
+DX = C.c_longdouble
+PDX = C.POINTER(C.c_longdouble)
 PI64 = C.POINTER(C.c_int64)
 U64 = C.c_uint64
 
@@ -28,6 +33,59 @@ class Params(C.Structure):
                 ("P", D), ("Ufr", D * 5), ("source", D * 5), ("eta", D)]
 
 
+class ParamsX(C.Structure):
+    """``parameters`` of the extended-precision build (long double fields)."""
+    _fields_ = [("gamma", DX), ("R", DX), ("mu", DX), ("c", DX), ("T", DX),
+                ("P", DX), ("Ufr", DX * 5), ("source", DX * 5), ("eta", DX)]
+
+
+def _register(L, D, PD, P):
+    L.orc_gauss_legendre.argtypes = [D, D, C.c_int, PD, PD]
+    L.orc_gauss_legendre.restype = C.c_int
+    L.orc_lagrange1d.argtypes = [C.c_int, C.c_int, D]
+    L.orc_lagrange1d.restype = D
+    L.orc_lagrange1d_deriv.argtypes = [C.c_int, C.c_int, D]
+    L.orc_lagrange1d_deriv.restype = D
+    L.orc_det3x3.argtypes = [PD]
+    L.orc_det3x3.restype = D
+    L.orc_inv3x3.argtypes = [PD, PD]
+    L.orc_matmul.argtypes = [C.c_int, C.c_int, C.c_int, D, PD, PD, D, PD]
+    L.orc_symmstore_mvmul.argtypes = [C.c_int, PD, PD, D, PD]
+    L.orc_spmm.argtypes = [C.c_int, C.c_int, PD, PD, D, PD]
+    L.orc_invert.argtypes = [PD, C.c_int]
+    L.orc_invert.restype = C.c_int
+    L.orc_analytical_flux.argtypes = [PD, D, PD, PD, PD]
+    L.orc_roe.argtypes = [PD, PD, PD, D, PD, PD]
+    L.orc_A.argtypes = [PD, PD, PD, PD, D, PD, PD, PD]
+    L.orc_bound_state.argtypes = [C.c_int64, PD, PD, C.POINTER(P), D, D, PD]
+    L.orc_bound_state.restype = C.c_int
+    L.orc_geom_create.argtypes = [C.c_int, C.c_int, U64, U64, U64, PI64, PD]
+    L.orc_geom_create.restype = C.c_void_p
+    L.orc_geom_destroy.argtypes = [C.c_void_p]
+    L.orc_geom_sizes.argtypes = [C.c_void_p, C.POINTER(U64)]
+    for nm in ("vJ", "Minv", "vgrad", "fgrad", "n", "fJ", "ih", "bh", "vw",
+               "fw", "veval", "feval"):
+        fn = getattr(L, "orc_geom_" + nm)
+        fn.argtypes = [C.c_void_p]
+        fn.restype = PD
+    for nm in ("ifl", "bfl"):
+        fn = getattr(L, "orc_geom_" + nm)
+        fn.argtypes = [C.c_void_p]
+        fn.restype = PI64
+    L.orc_residual.argtypes = [C.c_void_p, C.POINTER(P), PD, PD, PD]
+    L.orc_residual.restype = C.c_int
+    L.orc_step.argtypes = [C.c_void_p, C.POINTER(P), C.c_int, D, PD, PD]
+    L.orc_step.restype = C.c_int
+    L.orc_residual_norm.argtypes = [PD, U64]
+    L.orc_residual_norm.restype = D
+    L.orc_set_plant.argtypes = [C.c_int]
+    L.orc_eval_output.argtypes = [C.c_int, U64, PD, C.c_int, C.c_int, D, PD]
+    L.orc_update_mean.argtypes = [PD, PD, U64, U64, U64]
+    L.orc_turbulent.argtypes = [C.c_void_p, C.POINTER(P), D, PD]
+    L.RHS = C.CFUNCTYPE(None, PD, PD, C.c_void_p)
+    L.orc_step_generic.argtypes = [C.c_int, D, U64, PD, L.RHS, C.c_void_p]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -35,52 +93,20 @@ def lib():
             from oracle import build as _b
             _b.build()
         _lib = C.CDLL(LIB_PATH)
-        L = _lib
-        L.orc_gauss_legendre.argtypes = [D, D, C.c_int, PD, PD]
-        L.orc_gauss_legendre.restype = C.c_int
-        L.orc_lagrange1d.argtypes = [C.c_int, C.c_int, D]
-        L.orc_lagrange1d.restype = D
-        L.orc_lagrange1d_deriv.argtypes = [C.c_int, C.c_int, D]
-        L.orc_lagrange1d_deriv.restype = D
-        L.orc_det3x3.argtypes = [PD]
-        L.orc_det3x3.restype = D
-        L.orc_inv3x3.argtypes = [PD, PD]
-        L.orc_matmul.argtypes = [C.c_int, C.c_int, C.c_int, D, PD, PD, D, PD]
-        L.orc_symmstore_mvmul.argtypes = [C.c_int, PD, PD, D, PD]
-        L.orc_spmm.argtypes = [C.c_int, C.c_int, PD, PD, D, PD]
-        L.orc_invert.argtypes = [PD, C.c_int]
-        L.orc_invert.restype = C.c_int
-        L.orc_analytical_flux.argtypes = [PD, D, PD, PD, PD]
-        L.orc_roe.argtypes = [PD, PD, PD, D, PD, PD]
-        L.orc_A.argtypes = [PD, PD, PD, PD, D, PD, PD, PD]
-        L.orc_bound_state.argtypes = [C.c_int64, PD, PD, C.POINTER(Params), D, D, PD]
-        L.orc_bound_state.restype = C.c_int
-        L.orc_geom_create.argtypes = [C.c_int, C.c_int, U64, U64, U64, PI64, PD]
-        L.orc_geom_create.restype = C.c_void_p
-        L.orc_geom_destroy.argtypes = [C.c_void_p]
-        L.orc_geom_sizes.argtypes = [C.c_void_p, C.POINTER(U64)]
-        for nm in ("vJ", "Minv", "vgrad", "fgrad", "n", "fJ", "ih", "bh", "vw",
-                   "fw", "veval", "feval"):
-            fn = getattr(L, "orc_geom_" + nm)
-            fn.argtypes = [C.c_void_p]
-            fn.restype = PD
-        for nm in ("ifl", "bfl"):
-            fn = getattr(L, "orc_geom_" + nm)
-            fn.argtypes = [C.c_void_p]
-            fn.restype = PI64
-        L.orc_residual.argtypes = [C.c_void_p, C.POINTER(Params), PD, PD, PD]
-        L.orc_residual.restype = C.c_int
-        L.orc_step.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int, D, PD, PD]
-        L.orc_step.restype = C.c_int
-        L.orc_residual_norm.argtypes = [PD, U64]
-        L.orc_residual_norm.restype = D
-        L.orc_set_plant.argtypes = [C.c_int]
-        L.orc_eval_output.argtypes = [C.c_int, U64, PD, C.c_int, C.c_int, D, PD]
-        L.orc_update_mean.argtypes = [PD, PD, U64, U64, U64]
-        L.orc_turbulent.argtypes = [C.c_void_p, C.POINTER(Params), D, PD]
-        L.RHS = C.CFUNCTYPE(None, PD, PD, C.c_void_p)
-        L.orc_step_generic.argtypes = [C.c_int, D, U64, PD, L.RHS, C.c_void_p]
+        _register(_lib, D, PD, Params)
     return _lib
+
+
+def libx():
+    """The extended-precision build (same algorithm, long double)."""
+    global _libx
+    if _libx is None:
+        if not os.path.exists(LIBX_PATH):
+            from oracle import build as _b
+            _b.build()
+        _libx = C.CDLL(LIBX_PATH)
+        _register(_libx, DX, PDX, ParamsX)
+    return _libx
 
 
 def _p(a):
@@ -194,15 +220,21 @@ def bound_state(bt, UL, n, prm, Pt=0.0, Tt=0.0):
 
 class Geometry:
     """Reference geometry with its full precompute (core_geometry +
-    shared_geometry), built from element nodes."""
+    shared_geometry), built from element nodes.  ``extended=True`` uses the
+    extended-precision build: inputs are taken as given (doubles), every
+    operation runs in long double and results come back as np.longdouble."""
 
-    def __init__(self, p, q, nx, ny, nz, bounds, nodes):
-        L = lib()
+    def __init__(self, p, q, nx, ny, nz, bounds, nodes, extended=False):
+        self.extended = bool(extended)
+        self._L = libx() if self.extended else lib()
+        self._dt = np.longdouble if self.extended else np.float64
+        self._PD = PDX if self.extended else PD
+        L = self._L
         self.p, self.q, self.nx, self.ny, self.nz = p, q, nx, ny, nz
         self.bounds = np.asarray(bounds, dtype=np.int64)
-        self.nodes = _f64(nodes)
+        self.nodes = self._in(nodes)
         self.h = L.orc_geom_create(p, q, nx, ny, nz,
-                                   self.bounds.ctypes.data_as(PI64), _p(self.nodes))
+                                   self.bounds.ctypes.data_as(PI64), self._ptr(self.nodes))
         if not self.h:
             raise ValueError("oracle geometry construction failed")
         s = (U64 * 8)()
@@ -210,13 +242,24 @@ class Geometry:
         (self.nelem, self.nbf, self.nvq, self.nfq, self.nface, self.nifc,
          self.nbfc, self.nminv) = [int(v) for v in s]
 
+    def _in(self, a):
+        return np.ascontiguousarray(a, dtype=self._dt)
+
+    def _ptr(self, a):
+        return a.ctypes.data_as(self._PD)
+
+    def _prm(self, prm):
+        return params_from(prm, self.extended)
+
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_geom_destroy(self.h)
+            self._L.orc_geom_destroy(self.h)
             self.h = None
 
-    def _arr(self, name, count, dtype=np.float64):
-        ptr = getattr(lib(), "orc_geom_" + name)(self.h)
+    def _arr(self, name, count, dtype=None):
+        ptr = getattr(self._L, "orc_geom_" + name)(self.h)
+        if dtype is None:
+            dtype = np.float64
         return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype).copy()
 
     @property
@@ -293,28 +336,30 @@ class Geometry:
         return self.nelem * 5 * self.nbf
 
     def residual(self, prm, U):
-        U = _f64(U)
-        R = np.zeros(self.state_size())
-        f = np.zeros(self.state_size())
-        st = lib().orc_residual(self.h, C.byref(prm), _p(U), _p(R), _p(f))
+        U = self._in(U)
+        R = np.zeros(self.state_size(), dtype=self._dt)
+        f = np.zeros(self.state_size(), dtype=self._dt)
+        st = self._L.orc_residual(self.h, C.byref(self._prm(prm)), self._ptr(U),
+                                  self._ptr(R), self._ptr(f))
         if st:
             raise RuntimeError("oracle residual failed (status %d)" % st)
         return R, f
 
     def turbulent(self, prm, l):
         """turbulent (solver_init.cpp:30-73) on this mesh; returns U."""
-        U = np.zeros(self.state_size())
-        lib().orc_turbulent(self.h, C.byref(prm), l, _p(U))
+        U = np.zeros(self.state_size(), dtype=self._dt)
+        self._L.orc_turbulent(self.h, C.byref(self._prm(prm)), l, self._ptr(U))
         return U
 
     def step(self, prm, U, dt, scheme="RK4", nsteps=1):
         """Advance a copy of U by nsteps; returns (U, per-step RMS(R) norms)."""
-        U = _f64(U).copy()
+        U = self._in(U).copy()
         sc = 0 if scheme == "RK4" else 1
         norms = []
-        nrm = np.zeros(1)
+        nrm = np.zeros(1, dtype=self._dt)
+        cp = self._prm(prm)
         for _ in range(nsteps):
-            st = lib().orc_step(self.h, C.byref(prm), sc, dt, _p(U), _p(nrm))
+            st = self._L.orc_step(self.h, C.byref(cp), sc, dt, self._ptr(U), self._ptr(nrm))
             if st:
                 raise RuntimeError("oracle step failed (status %d)" % st)
             norms.append(nrm[0])
@@ -353,9 +398,10 @@ class planted:
         lib().orc_set_plant(0)
 
 
-def params_from(prm):
-    """oracle Params from any object with the reference ``parameters`` fields."""
-    o = Params()
+def params_from(prm, extended=False):
+    """oracle Params (ParamsX for the extended build) from any object with
+    the reference ``parameters`` fields."""
+    o = ParamsX() if extended else Params()
     for f in ("gamma", "R", "mu", "c", "T", "P", "eta"):
         setattr(o, f, getattr(prm, f))
     for i in range(5):
