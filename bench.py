@@ -1,24 +1,29 @@
 #!/usr/bin/env python3
 """Throughput benchmark of the gfx950 DG residual + explicit RK path.
 
-Workload (default): the 48x24x24 channel, p = 3 (64 DOF/element), q = 1 with
-smoothly displaced interior nodes (non-affine elements), periodic in x and z,
-no-slip walls at y = +-1, Mach 0.2, Re 3000, TVD-RK3 at CFL 0.3; synthetic
-seeded Poiseuille + divergence-free noise initial state.  One "step" is one
-TVD-RK3 time step (three fused residual + stage kernels) over the whole mesh,
-with the state resident in HBM.
+Workload (default): the 64^3 periodic Taylor-Green cube, p = 3 (64 DOF per
+element, 16.8 M DOF), affine q = 1 elements, Mach 0.1, Re 1600, classic RK4,
+seeded initial state.  One "step" is one RK4 time step (four fused residual
++ stage kernels) over the whole mesh, with the state resident in HBM, plus
+the RMS(R) of its first stage computed and kept on the device, as the
+reference's step functions compute it every step.
 
 Prints one JSON line:
   value      = DOF * stages * steps / time / 1e9            [GDOF-stage/s]
   roofline   = algorithmic HBM bytes per stage-kernel launch / average launch
-               time (HIP events on the library's stream) vs 8 TB/s
-  mixed      = the same workload in the mixed-precision form (FP32 face trace
-               records): throughput, speedup over FP64, per-field L2/Linf
-               errors and kinetic-energy dissipation rate against the FP64 run
+               time (HIP events on the library's stream, steps without the
+               norm) vs 8 TB/s
+  mixed      = the same workload in the mixed-precision form: throughput,
+               speedup over FP64, and after the workload's 100 RK4 steps
+               from the initial state per-field L2/Linf errors and the
+               kinetic-energy dissipation rate against the FP64 run
   cpu_baseline = the oracle (CPU restatement of the reference's dense
-               residual + RK) on a bounded sample, timed on this host's cores
+               residual + RK4) on a bounded sample of the same workload,
+               one 8^3 block per host thread
 
-Only one GPU is used: --gpus N > 1 exits with a message.
+--workload config0/config1 run the 8^3 p=2 cube and the curved channel the
+same way; config3 the 64-case ensemble, config4 the adjoint.  Only one GPU
+is used: --gpus N > 1 exits with a message.
 """
 import argparse
 import ctypes as C
@@ -26,6 +31,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -47,45 +54,67 @@ def hip():
     lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     lib.hipFree.argtypes = [C.c_void_p]
     lib.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
     return lib
 
 
-def alg_bytes_per_dof_stage(case):
+def alg_bytes_per_dof_stage(case, affine):
     """Compulsory HBM bytes per DOF per stage (FP64, 5 variables).
 
     Each full-state array costs 40 B/DOF.  TVD-RK3 fused stages read/write
     (U->U1), (U1,U->U2), (U2,U->U): 8 arrays over 3 stages.  Classic RK4 with
     an accumulator: (U->U1,acc), (U1,U,acc->U2,acc), (U2,U,acc->U3,acc),
     (U3,U,acc->U): 17 arrays over 4 stages.  Curved elements add their nodes
-    (24 doubles per q=1 element) once per stage."""
+    (24 doubles per q=1 element) once per stage; affine meshes have a
+    constant metric per element and add nothing."""
     arrays = 8.0 / 3.0 if case.scheme == "tvdRK3" else 17.0 / 4.0
     nq = case.mesh.q + 1
-    geom = 8.0 * 3 * nq ** 3 / case.mesh.nbf
+    geom = 0.0 if affine else 8.0 * 3 * nq ** 3 / case.mesh.nbf
     return 40.0 * arrays + geom
 
 
-def cpu_baseline(seconds_hint=20.0):
-    """Oracle timing on a bounded sample of the same workload: TVD-RK3 steps
-    of a 12x24x6 block of the p=3 curved channel (same per-element work)."""
+def cpu_baseline(seconds_hint=20.0, threads=None):
+    """The oracle (the CPU restatement of the reference's dense residual and
+    RK4) on a bounded sample of the 64^3 p=3 Taylor-Green workload: one
+    independent 8^3 p=3 periodic Taylor-Green block (same per-element work,
+    same physics) per host thread, RK4 steps at the workload's dt, all
+    threads concurrently (the oracle's C calls release the GIL).  Setup (the
+    oracle's per-element mass-matrix inversion) is timed apart."""
+    import threading
+
     from oracle import oracle as orc
-    c = cases.channel_case(nx=12, ny=24, nz=6, p=3, seed=1)
+    nthr = threads or min(os.cpu_count() or 1, 16)
+    c = cases.tgv_case(n=8, p=3, seed=2, dt=2.5e-4, nsteps=100, scheme="RK4")
     m = c.mesh
-    g = orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
     prm = orc.params_from(c.params)
-    U = c.U0.copy()
     t0 = time.perf_counter()
-    nsteps = 0
-    while True:
-        U, _ = g.step(prm, U, c.dt, c.scheme, 1)
-        nsteps += 1
-        el = time.perf_counter() - t0
-        if el > seconds_hint or nsteps >= 8:
-            break
-    val = c.dof * c.stages * nsteps / el / 1e9
-    return {"value": val, "unit": "GDOF-stage/s", "cores": 1, "kind": "port",
-            "sample": "%d TVD-RK3 step(s) of a 12x24x6 p=3 curved channel block "
-                      "(1728 elements, 110592 DOF), oracle on 1 host core, %.1f s"
-                      % (nsteps, el)}
+    geos = [orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes) for _ in range(nthr)]
+    setup = (time.perf_counter() - t0) / nthr
+    steps = [0] * nthr
+    stop = [False]
+
+    def work(i):
+        U = c.U0.copy()
+        while not stop[0]:
+            U, _ = geos[i].step(prm, U, c.dt, "RK4", 1)
+            steps[i] += 1
+
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(nthr)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    time.sleep(seconds_hint)
+    stop[0] = True
+    for t in ths:
+        t.join()
+    el = time.perf_counter() - t0
+    total_steps = sum(steps)
+    val = c.dof * 4 * total_steps / el / 1e9
+    return {"value": val, "unit": "GDOF-stage/s", "cores": nthr, "kind": "port",
+            "sample": "%d RK4 steps in total of %d independent 8^3 p=3 periodic Taylor-Green "
+                      "blocks (512 elements, 32768 DOF each, dt 2.5e-4), one per host thread, "
+                      "oracle (CPU restatement of the reference), %.1f s wall; setup %.2f s "
+                      "per block not included" % (total_steps, nthr, el, setup)}
 
 
 def run_ensemble(a, h):
@@ -316,39 +345,31 @@ def kinetic_rate(s, case):
     return k0, (k0 - k1) / case.dt
 
 
-def run_mixed(a, h, s, case, U64, value64, stream, ev0, ev1):
-    """The mixed-precision form (FP32 face trace records) on the same
-    workload: the same untimed warmup, tuning and step, the same timed
-    steps, then per-field L2/Linf errors against the FP64 run (both from the
-    same initial state after the same number of steps) and the kinetic-
-    energy dissipation rate of both forms over one more step."""
-    np = dgx.np
-    total = a.warmup + 1 + a.steps
-    # FP64 reference quantities at the end of the timed run
-    k64, eps64 = kinetic_rate(s, case)
-    s.set_state(case.U0)
-    s.precision = "mixed"
-    s.step_async(case.dt, case.scheme, a.warmup)
-    s.sync()
-    if a.autotune:
-        waves, _ = s.autotune(reps=3)
-    else:
-        waves = s.stage_waves
-    s.step_async(case.dt, case.scheme, 1)
-    s.sync()
+def timed_steps(h, s, case, nsteps, stream, ev0, ev1, norms_dev=None):
+    """GPU seconds of nsteps steps enqueued back to back (HIP events on the
+    library's stream); with norms_dev, every step also stores its RMS(R) on
+    the device as the reference's step functions compute it."""
     h.hipDeviceSynchronize()
     h.hipEventRecord(ev0, stream)
-    s.step_async(case.dt, case.scheme, a.steps)
+    if norms_dev is not None:
+        s.step_norms_async(case.dt, case.scheme, nsteps, norms_dev)
+    else:
+        s.step_async(case.dt, case.scheme, nsteps)
     h.hipEventRecord(ev1, stream)
     h.hipEventSynchronize(ev1)
     s.sync()
     ms = C.c_float()
     h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
-    gpu_s = ms.value * 1e-3
-    value = case.dof * case.stages * a.steps / gpu_s / 1e9
-    Um = s.get_state()
-    kmx, epsmx = kinetic_rate(s, case)
-    s.precision = "fp64"
+    return ms.value * 1e-3
+
+
+def read_norms(h, norms_dev, n):
+    out = np.zeros(n)
+    h.hipMemcpy(out.ctypes.data_as(C.c_void_p), norms_dev, 8 * n, 2)  # device to host
+    return out
+
+
+def field_errors(Um, U64, case, nsteps, k64, eps64, kmx, epsmx):
     A = Um.reshape(case.mesh.nelem, 5, -1)
     B = U64.reshape(case.mesh.nelem, 5, -1)
     fields = {}
@@ -360,22 +381,53 @@ def run_mixed(a, h, s, case, U64, value64, stream, ev0, ev1):
             "l2_rel_to_state": float(np.linalg.norm(d) / np.linalg.norm(U64)),
         }
     return {
+        "steps": nsteps,
+        "scheme": case.scheme,
+        "state_l2_rel": float(np.linalg.norm(Um - U64) / np.linalg.norm(U64)),
+        "fields": fields,
+        "kinetic_energy_rel": abs(kmx - k64) / abs(k64),
+        "ke_dissipation_rate_fp64": eps64,
+        "ke_dissipation_rate_mixed": epsmx,
+        "ke_dissipation_rate_rel": abs(epsmx - eps64) / abs(eps64),
+    }
+
+
+def run_mixed(a, h, s, case, value64, stream, ev0, ev1, norms_dev):
+    """The mixed-precision form on the same workload: the same untimed
+    warmup and tuning and the same timed steps (with per-step norms) from the
+    initial state; then the error leg: the workload's own step count (100
+    RK4 steps for the 64^3 case) from the initial state in both forms,
+    per-field L2/Linf errors of the mixed state against the FP64 state and
+    the kinetic-energy dissipation rate of both over one more step."""
+    s.set_state(case.U0)
+    s.precision = "mixed"
+    s.step_async(case.dt, case.scheme, a.warmup)
+    s.sync()
+    waves = s.autotune(reps=3)[0] if a.autotune else s.stage_waves
+    s.set_state(case.U0)
+    gpu_s = timed_steps(h, s, case, a.steps, stream, ev0, ev1, norms_dev)
+    value = case.dof * case.stages * a.steps / gpu_s / 1e9
+    gpu_s_nn = timed_steps(h, s, case, a.steps, stream, ev0, ev1)
+    # error leg
+    nerr = case.nsteps
+    s.set_state(case.U0)
+    s.step(case.dt, case.scheme, nerr, norms=False)
+    Um = s.get_state()
+    kmx, epsmx = kinetic_rate(s, case)
+    s.precision = "fp64"
+    s.set_state(case.U0)
+    s.step(case.dt, case.scheme, nerr, norms=False)
+    U64 = s.get_state()
+    k64, eps64 = kinetic_rate(s, case)
+    return {
         "value": value,
         "unit": "GDOF-stage/s",
         "speedup_vs_fp64": value / value64,
         "ms_per_step": 1e3 * gpu_s / a.steps,
-        "avg_launch_us": gpu_s / (a.steps * case.stages) * 1e6,
+        "avg_launch_us": gpu_s_nn / (a.steps * case.stages) * 1e6,
         "stage_waves_per_simd": waves,
-        "form": "FP32 face trace records; geometry, fluxes, Roe, inverse mass, RK in FP64",
-        "error_vs_fp64": {
-            "steps": total,
-            "state_l2_rel": float(np.linalg.norm(Um - U64) / np.linalg.norm(U64)),
-            "fields": fields,
-            "kinetic_energy_rel": abs(kmx - k64) / abs(k64),
-            "ke_dissipation_rate_fp64": eps64,
-            "ke_dissipation_rate_mixed": epsmx,
-            "ke_dissipation_rate_rel": abs(epsmx - eps64) / abs(eps64),
-        },
+        "form": s.MIXED_FORM,
+        "error_vs_fp64": field_errors(Um, U64, case, nerr, k64, eps64, kmx, epsmx),
     }
 
 
@@ -384,10 +436,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="config1",
+    ap.add_argument("--workload", default="config2",
                     choices=["config0", "config1", "config2", "config3", "config4"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default occupancy target instead of timing all of them")
     ap.add_argument("--no-mixed", action="store_true",
@@ -396,8 +449,6 @@ def main():
                     help="also run one copy kernel of this size (traffic calibration)")
     a = ap.parse_args()
     a.steps_given = a.steps is not None
-    if a.steps is None:
-        a.steps = 200
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build runs the path on a single "
               "MI355X; --gpus %d is not supported" % a.gpus, file=sys.stderr)
@@ -409,6 +460,9 @@ def main():
     if a.workload == "config4":
         return run_adjoint(a, h)
     case = {"config0": cases.config0, "config1": cases.config1, "config2": cases.config2}[a.workload]()
+    if a.steps is None:
+        a.steps = case.nsteps
+    affine = case.mesh.affine
     s = dgx.Solver(case.mesh, case.params)
     s.set_state(case.U0)
     stream = C.c_void_p(s.stream())
@@ -419,32 +473,29 @@ def main():
         waves, tune_us = s.autotune(reps=3)
     else:
         waves, tune_us = s.stage_waves, None
-    s.step_async(case.dt, case.scheme, 1)
-    s.sync()
-    h.hipDeviceSynchronize()
-
+    norms_dev = C.c_void_p()
+    h.hipMalloc(C.byref(norms_dev), 8 * a.steps)
     ev0, ev1 = C.c_void_p(), C.c_void_p()
     h.hipEventCreate(C.byref(ev0))
     h.hipEventCreate(C.byref(ev1))
-    t0 = time.perf_counter()
-    h.hipEventRecord(ev0, stream)
-    s.step_async(case.dt, case.scheme, a.steps)
-    h.hipEventRecord(ev1, stream)
-    h.hipEventSynchronize(ev1)
-    s.sync()
-    h.hipDeviceSynchronize()
-    wall = time.perf_counter() - t0
-    ms = C.c_float()
-    h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
-    gpu_s = ms.value * 1e-3
 
-    # the timed region holds exactly steps*stages stage-kernel launches
+    # the timed run: steps from the initial state, each with its RMS(R)
+    s.set_state(case.U0)
+    t0 = time.perf_counter()
+    gpu_s = timed_steps(h, s, case, a.steps, stream, ev0, ev1, norms_dev)
+    wall = time.perf_counter() - t0
+    norms = read_norms(h, norms_dev, a.steps)
+    U = s.get_state()
+    finite = bool(np.isfinite(U).all()) and bool(np.isfinite(norms).all())
+    value = case.dof * case.stages * a.steps / gpu_s / 1e9
+    # the same steps without the norm: stage-kernel launches only, which
+    # gives the stage kernel's average launch time
+    gpu_s_nn = timed_steps(h, s, case, a.steps, stream, ev0, ev1)
     launches = a.steps * case.stages
-    avg_launch_s = gpu_s / launches
-    bpd = alg_bytes_per_dof_stage(case)
+    avg_launch_s = gpu_s_nn / launches
+    bpd = alg_bytes_per_dof_stage(case, affine)
     bytes_per_launch = bpd * case.dof
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    value = case.dof * case.stages * a.steps / gpu_s / 1e9
 
     if a.calibrate_copy_mib:
         nb = a.calibrate_copy_mib << 20
@@ -456,14 +507,13 @@ def main():
         h.hipFree(src)
         h.hipFree(dst)
 
-    U = s.get_state()
-    finite = bool(dgx.np.isfinite(U).all())
-    mixed = None if a.no_mixed else run_mixed(a, h, s, case, U, value, stream, ev0, ev1)
+    mixed = None if a.no_mixed else run_mixed(a, h, s, case, value, stream, ev0, ev1, norms_dev)
+    h.hipFree(norms_dev)
     s.close()
 
     # HBM bytes and executed FP64 flops per stage-kernel launch, from the
     # committed rocprofv3 counter summary of the same kernel and workload
-    kname = "dgx_rhs_stage_kernel<%d,%d,%d>" % (case.mesh.p, case.mesh.q, waves)
+    kname = s.stage_kernel_name(waves)
     traffic = flops = None
     cf = os.path.join(ROOT, "profiles", "counters_%s.json" % a.workload)
     if os.path.exists(cf):
@@ -487,6 +537,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded mesh and initial state)",
         "state_finite": finite,
+        "norm_first_last": [float(norms[0]), float(norms[-1])],
         "config": {
             "workload": case.name,
             "description": {
@@ -495,10 +546,16 @@ def main():
                 "config2": "64^3 periodic Taylor-Green p=3, RK4",
             }[a.workload],
             "nelem": case.mesh.nelem, "p": case.mesh.p, "q": case.mesh.q,
-            "dof": case.dof, "scheme": case.scheme, "dt": case.dt,
+            "dof": case.dof, "scheme": case.scheme, "dt": case.dt, "affine": affine,
+            "timed": "%d steps from the initial state, each with the RMS(R) of its first "
+                     "stage kept on the device" % a.steps,
             "parallelism": "single GPU",
             "stage_waves_per_simd": waves,
             "autotune_us": None if tune_us is None else [round(float(x), 2) for x in tune_us],
+        },
+        "without_norm": {
+            "value": case.dof * case.stages * a.steps / gpu_s_nn / 1e9,
+            "ms_per_step": 1e3 * gpu_s_nn / a.steps,
         },
         "roofline": {
             "bound": "hbm",
@@ -525,7 +582,7 @@ def main():
     if mixed is not None:
         out["mixed"] = mixed
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds, a.cpu_threads)
     print(json.dumps(out), flush=True)
 
 

@@ -257,6 +257,15 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
     }
   }
   const bool store_all = store != nullptr;
+  if (!store_all && c->adj_store)
+  {
+    // this call checkpoints: a held store it does not use would only crowd
+    // the checkpoint buffers out of memory
+    AHIP(c, hipStreamSynchronize(c->stream));
+    AHIP(c, hipFree(c->adj_store));
+    c->adj_store = nullptr;
+    c->adj_store_bytes = 0;
+  }
   const uint64_t K = interval ? interval : (uint64_t)std::ceil(std::sqrt((double)nsteps));
   auto Sp = [&](uint64_t s, int i) { return store + (s * 4 + i) * (n + ntr); };
   auto Tp = [&](uint64_t s, int i) { return Sp(s, i) + n; };

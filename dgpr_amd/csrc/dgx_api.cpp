@@ -88,6 +88,10 @@ MeshArgs dgx::mesh_args(const dgx_ctx* c)
 // True when q = 1 and every element's bilinear and trilinear node terms
 // vanish to rounding (a parallelepiped, constant Jacobian); the stage
 // kernels then use one Jacobian per element instead of the geometry basis.
+// The twist terms are measured against the element's own size (its longest
+// edge-vector component), plus the rounding of coordinates far from the
+// origin (a few ulps of the largest coordinate), so a small element far from
+// the origin is not classified affine by its position alone.
 static bool mesh_is_affine(const dgx_mesh* m)
 {
   if (m->q != 1)
@@ -96,7 +100,7 @@ static bool mesh_is_affine(const dgx_mesh* m)
   for (uint64_t e = 0; e < ne; ++e)
   {
     const double* X = m->nodes + e * 24;
-    double twist = 0.0, scale = 0.0;
+    double twist = 0.0, size = 0.0, mag = 0.0;
     for (int d = 0; d < 3; ++d)
     {
       const double x000 = X[d], x100 = X[3 + d], x010 = X[6 + d], x110 = X[9 + d];
@@ -106,12 +110,23 @@ static bool mesh_is_affine(const dgx_mesh* m)
       twist = std::fmax(twist, std::fabs((x011 - x010) - (x001 - x000)));
       twist = std::fmax(twist,
                         std::fabs(((x111 - x011) - (x101 - x001)) - ((x110 - x010) - (x100 - x000))));
-      scale = std::fmax(scale, std::fmax(std::fabs(x000), std::fabs(x111)));
+      size = std::fmax(size, std::fmax(std::fabs(x100 - x000),
+                                       std::fmax(std::fabs(x010 - x000), std::fabs(x001 - x000))));
+      for (int c = 0; c < 8; ++c)
+        mag = std::fmax(mag, std::fabs(X[3 * c + d]));
     }
-    if (!(twist <= 1e-14 * scale))
+    if (!(twist <= 1e-14 * size + 8.0 * 2.220446049250313e-16 * mag))
       return false;
   }
   return true;
+}
+
+extern "C" dgx_status dgx_mesh_is_affine(const dgx_mesh* mesh, int* affine)
+{
+  if (!mesh || !affine || !mesh->nodes || mesh->q < 1)
+    return DGX_ERR_INVALID;
+  *affine = mesh_is_affine(mesh) ? 1 : 0;
+  return DGX_OK;
 }
 
 static dgx_status ensure_tables(int device)
@@ -499,8 +514,9 @@ extern "C" {
 
 // Launches one step; if want_norm, stores sum(R^2) of stage 1 in c->sum
 // (RK4: from the accumulator, which holds f1; TVD-RK3: from f written by the
-// first stage).
-static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm)
+// first stage), or, with norm_dev, the step's RMS(R) in *norm_dev.
+static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm,
+                              double* norm_dev = nullptr)
 {
   hipError_t e;
   if ((e = ensure_traces(c)) != hipSuccess)
@@ -521,7 +537,10 @@ static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm)
       if ((e = resid_out_dispatch(c->p, c->q, mesh_args(c), f1, nullptr, nullptr, c->partial,
                                   c->stream)) != hipSuccess)
         return e;
-      if ((e = sum_dispatch(c->partial, c->npartial, c->sum, c->stream)) != hipSuccess)
+      e = norm_dev ? norm_dispatch(c->partial, c->npartial, 1.0 / (double)c->nstate, norm_dev,
+                                   c->stream)
+                   : sum_dispatch(c->partial, c->npartial, c->sum, c->stream);
+      if (e != hipSuccess)
         return e;
     }
   }
@@ -543,6 +562,16 @@ dgx_status dgx_step_async(dgx_ctx* c, int scheme, double dt, uint64_t nsteps)
     return DGX_ERR_INVALID;
   for (uint64_t s = 0; s < nsteps; ++s)
     HIPCHK(c, launch_step(c, scheme, dt, false));
+  return DGX_OK;
+}
+
+dgx_status dgx_step_norms_async(dgx_ctx* c, int scheme, double dt, uint64_t nsteps,
+                                double* norms_dev)
+{
+  if (!c || !norms_dev || (scheme != DGX_RK4 && scheme != DGX_TVDRK3))
+    return DGX_ERR_INVALID;
+  for (uint64_t s = 0; s < nsteps; ++s)
+    HIPCHK(c, launch_step(c, scheme, dt, true, norms_dev + s));
   return DGX_OK;
 }
 

@@ -47,6 +47,7 @@ struct dgx_ensemble
   int scheme;
   int nstages;
   std::vector<double> dt;
+  std::vector<Phys> phys;  // each member's physics when the items were built
   bool graph_ready;
   hipGraph_t graph;
   hipGraphExec_t exec;
@@ -83,12 +84,24 @@ static void drop_graph(dgx_ensemble* en)
   }
 }
 
-// Builds the per-stage item tables of every group for (scheme, dt[]).
+// True when some member's physics changed (dgx_set_params on a member
+// context) since the item tables were built: the tables and a captured graph
+// carry a copy of it.
+static bool phys_changed(const dgx_ensemble* en)
+{
+  for (size_t i = 0; i < en->ctx.size(); ++i)
+    if (std::memcmp(&en->phys[i], &en->ctx[i]->phys, sizeof(Phys)) != 0)
+      return true;
+  return false;
+}
+
+// Builds the per-stage item tables of every group for (scheme, dt[]) and
+// the members' current physics.
 static dgx_status plan(dgx_ensemble* en, int scheme, const double* dt)
 {
   const int n = (int)en->ctx.size();
   if (en->planned && en->scheme == scheme &&
-      std::memcmp(en->dt.data(), dt, sizeof(double) * n) == 0)
+      std::memcmp(en->dt.data(), dt, sizeof(double) * n) == 0 && !phys_changed(en))
     return DGX_OK;
   drop_graph(en);
   int ns = 0;
@@ -123,6 +136,9 @@ static dgx_status plan(dgx_ensemble* en, int scheme, const double* dt)
   en->scheme = scheme;
   en->nstages = ns;
   en->dt.assign(dt, dt + n);
+  en->phys.resize(n);
+  for (int i = 0; i < n; ++i)
+    en->phys[i] = en->ctx[i]->phys;
   return DGX_OK;
 }
 

@@ -1108,8 +1108,11 @@ __global__ __launch_bounds__(64) void dgx_resid_out_kernel(MeshArgs mesh, const 
   }
 }
 
-// Deterministic sum of the per-block partial sums into out[0].
-__global__ __launch_bounds__(256) void dgx_sum_kernel(const double* partial, int n, double* out)
+// Deterministic sum of the per-block partial sums into out[0]; with
+// scale > 0, out[0] = sqrt(scale * sum) instead (the residual norm of
+// residual_norm, time_distributed.cpp:23-44, with scale = 1 / state length).
+__global__ __launch_bounds__(256) void dgx_sum_kernel(const double* partial, int n, double* out,
+                                                      double scale)
 {
   __shared__ double red[256];
   double t = 0.0;
@@ -1124,7 +1127,7 @@ __global__ __launch_bounds__(256) void dgx_sum_kernel(const double* partial, int
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    out[0] = red[0];
+    out[0] = (scale > 0.0) ? sqrt(scale * red[0]) : red[0];
 }
 
 // Plain streaming copy (16 B per lane per iteration).
@@ -1567,7 +1570,14 @@ hipError_t dot_dispatch(const double* x, const double* y, size_t n, double* part
 
 hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st)
 {
-  hipLaunchKernelGGL(dgx_sum_kernel, dim3(1), dim3(256), 0, st, partial, n, out);
+  hipLaunchKernelGGL(dgx_sum_kernel, dim3(1), dim3(256), 0, st, partial, n, out, 0.0);
+  return hipGetLastError();
+}
+
+hipError_t norm_dispatch(const double* partial, int n, double inv_count, double* out,
+                         hipStream_t st)
+{
+  hipLaunchKernelGGL(dgx_sum_kernel, dim3(1), dim3(256), 0, st, partial, n, out, inv_count);
   return hipGetLastError();
 }
 

@@ -113,6 +113,9 @@ hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg,
 hipError_t basis_dispatch(int p, const double* in, double* out, int nelem, int mode,
                           hipStream_t st);
 hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st);
+// *out = sqrt(inv_count * sum of the n partial sums): one step's residual norm
+hipError_t norm_dispatch(const double* partial, int n, double inv_count, double* out,
+                         hipStream_t st);
 // psiV = (d f_g / d V)^T psi on a periodic mesh, V with its trace records Tin;
 // pvol, aown, anbr are scratch (a state and two trace-record sets)
 hipError_t adjoint_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,

@@ -96,6 +96,7 @@ def lib():
         L.dgx_residual_device.argtypes = [P_ctx, C.c_void_p, C.c_void_p, C.c_void_p]
         L.dgx_step.argtypes = [P_ctx, C.c_int, D, U64, PD]
         L.dgx_step_async.argtypes = [P_ctx, C.c_int, D, U64]
+        L.dgx_step_norms_async.argtypes = [P_ctx, C.c_int, D, U64, C.c_void_p]
         L.dgx_device_copy.argtypes = [P_ctx, C.c_void_p, C.c_void_p, U64]
         L.dgx_sync.argtypes = [P_ctx]
         L.dgx_get_stream.argtypes = [P_ctx, C.POINTER(C.c_void_p)]
@@ -112,6 +113,7 @@ def lib():
         L.dgx_read_dg_header.argtypes = [C.c_char_p, C.POINTER(U64), C.POINTER(_CMesh)]
         L.dgx_read_dg.argtypes = [C.c_char_p, PD, PD]
         L.dgx_set_tuning.argtypes = [P_ctx, C.c_int]
+        L.dgx_mesh_is_affine.argtypes = [C.POINTER(_CMesh), C.POINTER(C.c_int)]
         L.dgx_kinetic_energy.argtypes = [P_ctx, PD]
         L.dgx_residual_vjp.argtypes = [P_ctx, PD, PD]
         L.dgx_adjoint_rk4.argtypes = [P_ctx, D, U64, U64, PD, PD, PD]
@@ -137,11 +139,13 @@ def lib():
         L.dgx_autotune.argtypes = [P_ctx, C.c_int, C.POINTER(C.c_int), PD]
         for nm in ("dgx_create", "dgx_destroy", "dgx_set_params", "dgx_set_state",
                    "dgx_get_state", "dgx_set_state_device", "dgx_get_state_device",
-                   "dgx_residual", "dgx_residual_device", "dgx_step", "dgx_step_async", "dgx_device_copy",
+                   "dgx_residual", "dgx_residual_device", "dgx_step", "dgx_step_async",
+                   "dgx_step_norms_async", "dgx_device_copy",
                    "dgx_sync", "dgx_get_stream", "dgx_get_sizes", "dgx_make_reference_state",
                    "dgx_bump_nodes", "dgx_box_nodes", "dgx_tgv_state", "dgx_channel_state",
                    "dgx_turbulent_state",
                    "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg", "dgx_set_tuning",
+                   "dgx_mesh_is_affine",
                    "dgx_get_tuning", "dgx_autotune", "dgx_set_precision", "dgx_get_precision",
                    "dgx_sample_output",
                    "dgx_sample_output_device", "dgx_update_mean", "dgx_get_mean",
@@ -200,6 +204,15 @@ class Mesh:
     @property
     def state_size(self):
         return self.nelem * 5 * self.nbf
+
+    @property
+    def affine(self):
+        """Whether libdgx runs its constant-Jacobian kernels on this mesh
+        (dgx_mesh_is_affine)."""
+        cm = self.cstruct()
+        a = C.c_int()
+        _chk(lib().dgx_mesh_is_affine(C.byref(cm), C.byref(a)))
+        return bool(a.value)
 
     def cstruct(self):
         m = _CMesh()
@@ -360,6 +373,13 @@ class Solver:
         sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
         _chk(lib().dgx_step_async(self.ctx, sc, dt, nsteps), self.ctx)
 
+    def step_norms_async(self, dt, scheme, nsteps, norms_ptr):
+        """Enqueue nsteps steps; step s writes its RMS(R) to the device
+        double at norms_ptr + 8 s (an integer address)."""
+        sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        _chk(lib().dgx_step_norms_async(self.ctx, sc, dt, nsteps, C.c_void_p(int(norms_ptr))),
+             self.ctx)
+
     def sync(self):
         _chk(lib().dgx_sync(self.ctx), self.ctx)
 
@@ -433,6 +453,14 @@ class Solver:
     @precision.setter
     def precision(self, name):
         _chk(lib().dgx_set_precision(self.ctx, PRECISIONS[name]), self.ctx)
+
+    # what DGX_PREC_MIXED computes in FP32 (the rest stays FP64)
+    MIXED_FORM = "FP32 face trace records; geometry, fluxes, Roe, inverse mass, RK in FP64"
+
+    def stage_kernel_name(self, waves=None):
+        """rocprof name prefix of the stage kernel this context launches."""
+        w = self.stage_waves if waves is None else waves
+        return "dgx_rhs_stage_kernel<%d,%d,%d>" % (self.mesh.p, self.mesh.q, w)
 
     def autotune(self, reps=5):
         """Times each compiled occupancy target on this mesh, keeps the

@@ -71,7 +71,10 @@ class ResidualFunction(torch.autograd.Function):
 
 class KineticEnergyRK4(torch.autograd.Function):
     """J(U0, mu) after ``nsteps`` RK4 steps of size ``dt``; ``interval`` is
-    the checkpoint spacing of the backward pass (0 picks about sqrt(n))."""
+    the checkpoint spacing of the backward pass: 0 keeps every stage state
+    from the forward sweep when they fit in three quarters of the free
+    device memory (nothing recomputed) and checkpoints every
+    ceil(sqrt(nsteps)) steps otherwise; k >= 1 checkpoints every k steps."""
 
     @staticmethod
     def forward(ctx, U0, mu, solver, dt, nsteps, interval=0):

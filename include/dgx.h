@@ -144,6 +144,15 @@ dgx_status dgx_step(dgx_ctx* ctx, int scheme, double dt, uint64_t nsteps,
 dgx_status dgx_step_async(dgx_ctx* ctx, int scheme, double dt,
                           uint64_t nsteps);
 
+/* Enqueue-only form of dgx_step with norms: the RMS of R at the first stage
+ * of step s (residual_norm, time_distributed.cpp:23-44, called by
+ * RK4_distributed / tvdRK3_distributed every step) goes to norms_dev[s], a
+ * device buffer of nsteps doubles, without a host round trip per step.  The
+ * caller reads them after dgx_sync and applies the reference's NaN check
+ * itself (dgx_step stops at the first NaN). */
+dgx_status dgx_step_norms_async(dgx_ctx* ctx, int scheme, double dt, uint64_t nsteps,
+                                double* norms_dev);
+
 /* Enqueues a device-to-device copy of bytes (a multiple of 16) on the
  * context's stream with a plain streaming kernel.  Replaces the reference's
  * state copies between device buffers (cudaMemcpy device-to-device is not
@@ -254,7 +263,8 @@ dgx_status dgx_residual_vjp(dgx_ctx* ctx, const double* w_host, double* g_host);
  * bit for bit.  The stored stages stay allocated on the context for later
  * calls (a buffer that is large enough is reused; freeing tens of GB can
  * take seconds) until dgx_adjoint_release or dgx_destroy; the free-memory
- * test counts the held buffer as free. */
+ * test counts the held buffer as free, and a call that checkpoints frees a
+ * held buffer before its own allocations. */
 dgx_status dgx_adjoint_rk4(dgx_ctx* ctx, double dt, uint64_t nsteps, uint64_t interval,
                            double* J, double* dJdU0_host, double* dJdmu);
 
@@ -315,6 +325,14 @@ dgx_status dgx_bump_nodes(uint64_t q, uint64_t nx, uint64_t ny, uint64_t nz,
                           double lx, double ly, double lz, double a,
                           double bmp_height, double bmp_center,
                           double bmp_variance, double* nodes_out);
+
+/* Whether dgx_create will run the constant-Jacobian kernels on this mesh
+ * (no counterpart in the reference, which stores per-point metrics for every
+ * element, geometry_common.cpp:292-395): *affine = 1 when q = 1 and every
+ * element is a parallelepiped, i.e. its bilinear and trilinear node terms
+ * are below 1e-14 of its longest edge-vector component plus 8 ulps of its
+ * largest coordinate. */
+dgx_status dgx_mesh_is_affine(const dgx_mesh* mesh, int* affine);
 
 /* Box nodes on [x0,x0+lx]x[y0,y0+ly]x[z0,z0+lz] with each node displaced by
  * a smooth field of relative amplitude amp (fraction of the local element

@@ -99,6 +99,9 @@ CASES = {
     # p=3 Mach 0.1 Taylor-Green at the CFL of the 64^3 workload (the
     # ill-conditioned case): residual, and 100 RK4 steps
     "tgv4_p3_long": lambda: cases.tgv_case(n=4, p=3, seed=25, dt=4e-3, nsteps=100),
+    # 8^3 p=3 Mach 0.1 Taylor-Green: the physics and order of the 64^3
+    # workload on 1/512 of its elements, at its CFL (dt c / h), 10 RK4 steps
+    "tgv8_p3": lambda: cases.tgv_case(n=8, p=3, seed=28, dt=2e-3, nsteps=10),
     "tgv_curved_p3": lambda: cases.tgv_case(nxyz=(3, 4, 2), p=3, seed=14, amp=0.15, nsteps=3),
     "tgv_curved_p2": lambda: cases.tgv_case(nxyz=(4, 3, 3), p=2, seed=24, amp=0.2, nsteps=3),
     "channel_p1": lambda: cases.channel_case(4, 4, 3, p=1, seed=15, nsteps=5),

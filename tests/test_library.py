@@ -178,3 +178,20 @@ def test_entry_points_reject_null_arguments():
     prm = c.params.copy()
     assert L.dgx_turbulent_state(C.byref(cm), C.byref(prm), 0.0, buf) == dgx.ERR_INVALID
     assert L.dgx_turbulent_state(C.byref(cm), None, 1.0, buf) == dgx.ERR_INVALID
+
+
+def test_affine_classifier_scales_with_element_size():
+    """dgx_mesh_is_affine measures an element's twist against its own size:
+    a box of small elements far from the origin (1e4, element size 1e-3) with
+    a twist of 5e-8 of the element size (25 ulps of the coordinates, 5e-15 of
+    their magnitude) is curved, the same box untwisted is affine, and q = 2
+    is never affine."""
+    n, h, x0 = 2, 1e-3, 1e4
+    nodes = dgx.box_nodes(1, n, n, n, x0, x0, x0, n * h, n * h, n * h, 0.0, 0)
+    straight = dgx.Mesh(2, 1, n, n, n, [dgx.PERIODIC] * 6, nodes)
+    assert straight.affine
+    twisted = nodes.reshape(-1, 8, 3).copy()
+    twisted[:, 7, 0] += 5e-8 * h  # corner (1,1,1) moved: a trilinear term
+    assert not dgx.Mesh(2, 1, n, n, n, [dgx.PERIODIC] * 6, twisted.reshape(-1)).affine
+    q2 = dgx.box_nodes(2, n, n, n, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 0.0, 0)
+    assert not dgx.Mesh(2, 2, n, n, n, [dgx.PERIODIC] * 6, q2).affine

@@ -1,7 +1,8 @@
 """Pins the oracle against the reference's own outputs.
 
-``tests/golden/reference_residuals.npz`` holds, for 22 cases (BASELINE
-config 0 at full size, periodic Taylor-Green p = 1..3, curved q = 1..3
+``tests/golden/reference_residuals.npz`` holds, for 23 cases (BASELINE
+config 0 at full size, periodic Taylor-Green p = 1..3 up to 8^3 p = 3 at
+Mach 0.1, curved q = 1..3
 channels with no-slip and inviscid walls, no-slip walls under a moving
 state, the reference's default bump with subsonic inflow / outflow and
 inviscid walls, free-stream boundaries with a source term, single-element
@@ -105,23 +106,25 @@ def test_boundary_plants_cannot_act_without_boundary_faces(orc):
         assert rel(f, c.f) < TOL
 
 
-@pytest.mark.parametrize("name", ["config0", "tgv4_p3_long", "tgv3_p3", "moving_wall_p3",
-                                  "channel_q3_p3"])
+@pytest.mark.parametrize("name", ["config0", "tgv4_p3_long", "tgv8_p3", "tgv3_p3",
+                                  "moving_wall_p3", "channel_q3_p3"])
 def test_extended_precision_floor(orc, name):
     """The double evaluations (reference and oracle) against the same
-    algorithm in extended precision.  Both stay within 2e-12 of the exact
-    discrete residual; the p = 3 Mach 0.1 Taylor-Green cases sit near 1e-12
-    (the reference's own f is that far from exact), the walled and moving
-    cases two orders lower."""
+    algorithm in extended precision.  The Mach 0.1 Taylor-Green cases sit at
+    1e-12 (4^3 p = 3, 8^3 p = 2) to 4e-12 (8^3 p = 3) from the exact
+    discrete residual (the reference's own f is that far from exact: the
+    floor grows with p and with 1/h), the walled and moving cases two orders
+    lower."""
     c, geo, prm = setup(orc, name)
     _, gx, _ = setup(orc, name, extended=True)
     Rx, fx = gx.residual(prm, c.U0)
     _, fo = geo.residual(prm, c.U0)
     e_ref, e_orc = rel(c.f, fx), rel(fo, fx)
     print("%s: reference vs exact %.2e, oracle vs exact %.2e" % (name, e_ref, e_orc))
-    assert e_ref < 2e-12 and e_orc < 2e-12
-    assert rel(c.R, Rx) < 1e-12
-    if name in ("config0", "tgv4_p3_long"):
+    bound = 6e-12 if name == "tgv8_p3" else 2e-12
+    assert e_ref < bound and e_orc < bound
+    assert rel(c.R, Rx) < (2e-12 if name == "tgv8_p3" else 1e-12)
+    if name in ("config0", "tgv4_p3_long", "tgv8_p3"):
         assert e_ref > 5e-13  # the ill-conditioned cases really are at the floor
     if name == "moving_wall_p3":
         assert e_ref < 1e-13
