@@ -29,6 +29,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -144,6 +145,41 @@ hipError_t copy_state(dgx_ctx* c, double* dst, const double* src)
 
 }  // namespace
 
+// ---- allocation-free form of the residual adjoint -------------------------
+// dgx_residual_vjp_device_noalloc takes its workspace (the basis-converted
+// cotangent, the volume-term partial adjoint, the two adjoint trace-record
+// sets and the Gauss-nodal result: 3 states + 2 record sets) from this
+// file-scope device array instead of hipMalloc.  Each device holds its own
+// copy for the life of the process; every call on a device shares it.
+constexpr size_t kScratchDoubles = (size_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace
+{
+std::mutex g_scratch_mutex;
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on device dev, which must be current; looked
+// up once per device (the host-side name of the array is not its address).
+static hipError_t scratch_address(int dev, double** out)
+{
+  if (dev < 0 || dev >= kMaxDevices)
+    return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (!g_scratch_ptr[dev])
+  {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return e;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 extern "C" {
 
 dgx_status dgx_kinetic_energy(dgx_ctx* c, double* J)
@@ -183,6 +219,52 @@ dgx_status dgx_residual_vjp(dgx_ctx* c, const double* w_host, double* g_host)
   AHIP(c, hipMemcpyAsync(g_host, b.p, c->nstate * sizeof(double), hipMemcpyDeviceToHost,
                          c->stream));
   AHIP(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
+dgx_status dgx_residual_vjp_device_noalloc(dgx_ctx* c, const double* w_dev, double* g_dev)
+{
+  if (!c || !w_dev || !g_dev)
+    return DGX_ERR_INVALID;
+  if (!all_periodic(c))
+    return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs a periodic mesh");
+  if (c->prec != DGX_PREC_FP64)
+    return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs DGX_PREC_FP64");
+  const size_t n = c->nstate;
+  const size_t tr = trace_doubles(c->p, c->nx, c->ny, c->nz);
+  const size_t need = 3 * n + 2 * tr;
+  if (need > kScratchDoubles)
+    return afail(c, DGX_ERR_INVALID,
+                 "mesh too large for the allocation-free residual adjoint: it needs %zu bytes "
+                 "of workspace, the fixed scratch holds %zu (use dgx_residual_vjp)",
+                 need * sizeof(double), kScratchDoubles * sizeof(double));
+  int prev = -1;
+  AHIP(c, hipGetDevice(&prev));
+  if (prev != c->device)
+    AHIP(c, hipSetDevice(c->device));
+  double* base = nullptr;
+  hipError_t e = scratch_address(c->device, &base);
+  double* b    = base;          // B3^-T w
+  double* pvol = base + n;      // volume-term partial adjoint
+  double* a    = base + 2 * n;  // (df_g/dV)^T B3^-T w
+  double* aown = base + 3 * n;  // adjoint trace records, own and neighbour side
+  double* anbr = aown + tr;
+  // f = B3^-1 f_g(B3 U): (df/dU)^T w = B3^T (df_g/dV)^T B3^-T w
+  if (e == hipSuccess)
+    e = ensure_traces(c);
+  if (e == hipSuccess)
+    e = basis_dispatch(c->p, w_dev, b, c->nelem, 3, c->stream);
+  if (e == hipSuccess)
+    e = adjoint_dispatch(c->p, c->q, mesh_args(c), c->phys, c->U, c->TU, c->lam, b, pvol, aown,
+                         anbr, a, c->stream);
+  if (e == hipSuccess)
+    e = basis_dispatch(c->p, a, g_dev, c->nelem, 2, c->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(c->stream);
+  if (prev != c->device)
+    (void)hipSetDevice(prev);
+  if (e != hipSuccess)
+    return afail(c, DGX_ERR_HIP, "residual adjoint (allocation-free): %s", hipGetErrorString(e));
   return DGX_OK;
 }
 

@@ -734,6 +734,22 @@ dgx_status dgx_update_mean(dgx_ctx* c, uint64_t time_step, uint64_t mean_start)
   return DGX_OK;
 }
 
+dgx_status dgx_set_mean(dgx_ctx* c, const double* mean_host)
+{
+  if (!c || !mean_host)
+    return DGX_ERR_INVALID;
+  size_t sb = c->nstate * sizeof(double);
+  if (!c->mean)
+  {
+    HIPCHK(c, hipMalloc(&c->mean, sb));
+    c->bytes += sb;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->Ua, mean_host, sb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, basis_dispatch(c->p, c->Ua, c->mean, c->nelem, 0, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DGX_OK;
+}
+
 dgx_status dgx_get_mean(dgx_ctx* c, double* mean_host)
 {
   if (!c || !mean_host)

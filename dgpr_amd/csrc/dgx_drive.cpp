@@ -5,7 +5,12 @@
 //   uniform (or -turb) state, or read "<ifile>.dg" and continue from the step
 //   after the one it holds; step with tvdRK3 or RK4 printing "tstep : norm";
 //   write "<ofile><tstep>.dg" whenever (tstep + 1) % chkint == 0 before that
-//   step, and "<ofile>.dg" at the end.
+//   step, and "<ofile>.dg" at the end, with every output the run holds: the
+//   outputs read from "<ifile>.dg" are carried through (as cudg.cu's
+//   outputs map is), and from step -mean_start on the running mean of the
+//   state is kept on the device and written as "mean" (the CPU driver's
+//   averaging, dg.cpp:122-131; a "mean" read from the input file is
+//   continued).
 // Errors print their message and exit with status 1, as errout does
 // (error.cpp:34-57).
 #include <cstdio>
@@ -37,7 +42,7 @@ int main(int argc, char** argv)
 
     u64 tstep = 0;
     core_geometry geom;
-    simstate U;
+    outputs_map outputs;
 
     if (inputs.ifile_prefix[0] == '\0')
     {
@@ -46,7 +51,7 @@ int main(int argc, char** argv)
       geom = bump_shared(inputs.sol_order, inputs.geo_order, inputs.nx, inputs.ny, inputs.nz,
                          inputs.lx, inputs.ly, inputs.lz, inputs.stretch, inputs.bmp_height,
                          inputs.bmp_center, inputs.bmp_variance, bounds);
-      U = simstate(geom);
+      simstate& U = outputs.add("state", simstate(geom));
       if (inputs.init_turbulent)
         turbulent(geom, sim_params, inputs.lx, U);
       else
@@ -56,9 +61,10 @@ int main(int argc, char** argv)
     {
       char ifile[1024];
       std::snprintf(ifile, sizeof(ifile), "%s.dg", inputs.ifile_prefix);
-      read_state(ifile, tstep, geom, U);
+      read_state(ifile, tstep, geom, outputs);
       tstep += 1;  // starting on the step after the one that was read
     }
+    simstate& U = outputs.get("state");
 
     /* data prep */
 
@@ -72,6 +78,15 @@ int main(int argc, char** argv)
 
     device_geometry cugeom(geom, sim_params);
     cugeom.upload(U);
+    if (outputs.has("mean"))
+      cugeom.upload_mean(outputs.get("mean"));
+
+    // outputs held on the device, brought to the host before each write
+    auto download = [&]() {
+      cugeom.download(outputs.get("state"));
+      if (outputs.has("mean"))
+        cugeom.download_mean(outputs.get("mean"));
+    };
 
     /* time stepping */
 
@@ -81,16 +96,25 @@ int main(int argc, char** argv)
     {
       if ((tstep + 1) % inputs.chkint == 0)
       {
-        cugeom.download(U);
+        download();
         std::snprintf(ofile, sizeof(ofile), "%s%" PRIu64 ".dg", inputs.ofile_prefix, tstep);
-        write_state(ofile, tstep, geom, U);
+        write_state(ofile, tstep, geom, outputs);
       }
       step(tstep, inputs.dt, cugeom);
+
+      // (re-)set the mean when averaging starts, then average
+      if (tstep == inputs.mean_start)
+        outputs.add("mean", simstate(geom));
+      if (tstep >= inputs.mean_start)
+      {
+        outputs.get("mean");  // a restart past mean_start needs the mean it read
+        cugeom.update_mean(tstep, inputs.mean_start);
+      }
     }
 
-    cugeom.download(U);
+    download();
     std::snprintf(ofile, sizeof(ofile), "%s.dg", inputs.ofile_prefix);
-    write_state(ofile, tstep - 1, geom, U);
+    write_state(ofile, tstep - 1, geom, outputs);
   }
   catch (const error& e)
   {

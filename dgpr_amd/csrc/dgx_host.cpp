@@ -1,5 +1,6 @@
 // dgx_host.cpp -- host-side mesh/state generators and the .dg checkpoint
 // format of libdgx (no device work).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -319,10 +320,37 @@ dgx_status dgx_channel_state(const dgx_mesh* m, double gamma, double rho0, doubl
 
 // ---- .dg files (io.cpp:33-159) ---------------------------------------------
 
-dgx_status dgx_write_dg(const char* path, uint64_t tstep, const dgx_mesh* m, const double* U)
+// Bucket of an output key in the reference's map<simstate> (djb2 hash mod
+// 256 table entries, data_structures.cpp:537-544, 670-673): write_state
+// visits the outputs in increasing bucket order, and within a bucket in the
+// order they were added (list<T>::add appends, data_structures.cpp:444-463).
+static uint32_t output_bucket(const char* key)
 {
-  if (!path || !mesh_ok(m) || !U)
+  uint32_t h = 5381;
+  for (const char* c = key; *c; ++c)
+    h = h * 33u + (uint32_t)(unsigned char)*c;
+  return h % 256u;
+}
+
+dgx_status dgx_write_dg_outputs(const char* path, uint64_t tstep, const dgx_mesh* m,
+                                uint64_t nout, const char* const* keys,
+                                const double* const* states)
+{
+  if (!path || !mesh_ok(m) || nout == 0 || !keys || !states)
     return DGX_ERR_INVALID;
+  std::vector<uint64_t> order;
+  for (uint64_t o = 0; o < nout; ++o)
+  {
+    if (!keys[o] || !states[o] || strlen(keys[o]) == 0 || strlen(keys[o]) >= 1024)
+      return DGX_ERR_INVALID;
+    for (uint64_t x = 0; x < o; ++x)
+      if (strcmp(keys[x], keys[o]) == 0)
+        return DGX_ERR_INVALID;  // the reference's map holds one output per key
+    order.push_back(o);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
+    return output_bucket(keys[a]) < output_bucket(keys[b]);
+  });
   FILE* f = fopen(path, "wb");
   if (!f)
     return DGX_ERR_IO;
@@ -336,20 +364,27 @@ dgx_status dgx_write_dg(const char* path, uint64_t tstep, const dgx_mesh* m, con
   w64(&m->q);
   for (int i = 0; i < 6; ++i)
     w64(&m->bounds[i]);
-  uint64_t nout = 1;
   w64(&nout);
   uint64_t nelem = m->nx * m->ny * m->nz;
   uint64_t nq = m->q + 1, np = m->p + 1;
   uint64_t nn = nelem * nq * nq * nq * 3;
   ok = ok && fwrite(m->nodes, sizeof(double), nn, f) == nn;
-  const char key[] = "state";
-  uint64_t klen = 5;
-  w64(&klen);
-  ok = ok && fwrite(key, 1, klen, f) == klen;
   uint64_t ns = nelem * 5 * np * np * np;
-  ok = ok && fwrite(U, sizeof(double), ns, f) == ns;
+  for (uint64_t o : order)
+  {
+    uint64_t klen = strlen(keys[o]);
+    w64(&klen);
+    ok = ok && fwrite(keys[o], 1, klen, f) == klen;
+    ok = ok && fwrite(states[o], sizeof(double), ns, f) == ns;
+  }
   ok = (fclose(f) == 0) && ok;
   return ok ? DGX_OK : DGX_ERR_IO;
+}
+
+dgx_status dgx_write_dg(const char* path, uint64_t tstep, const dgx_mesh* m, const double* U)
+{
+  const char* key = "state";
+  return dgx_write_dg_outputs(path, tstep, m, 1, &key, &U);
 }
 
 static dgx_status read_header(FILE* f, uint64_t* tstep, dgx_mesh* m, uint64_t* nout)
@@ -382,6 +417,88 @@ dgx_status dgx_read_dg_header(const char* path, uint64_t* tstep, dgx_mesh* m)
   dgx_status st = read_header(f, tstep, m, &nout);
   fclose(f);
   return st;
+}
+
+}  // extern "C"
+
+// Visits the outputs of a .dg file in file order: fn(index, key, f) with the
+// file positioned at the output's data, which fn reads or skips; returns
+// the header's output count in *nout.
+template <typename Fn>
+static dgx_status for_each_output(const char* path, double* nodes_out, uint64_t* nout, Fn fn)
+{
+  FILE* f = fopen(path, "rb");
+  if (!f)
+    return DGX_ERR_IO;
+  uint64_t tstep = 0;
+  dgx_mesh m;
+  dgx_status st = read_header(f, &tstep, &m, nout);
+  if (st != DGX_OK)
+  {
+    fclose(f);
+    return st;
+  }
+  uint64_t nelem = m.nx * m.ny * m.nz;
+  uint64_t nq = m.q + 1, np = m.p + 1;
+  uint64_t nn = nelem * nq * nq * nq * 3;
+  uint64_t ns = nelem * 5 * np * np * np;
+  bool ok = nodes_out ? fread(nodes_out, sizeof(double), nn, f) == nn
+                      : fseek(f, (long)(nn * sizeof(double)), SEEK_CUR) == 0;
+  for (uint64_t o = 0; ok && o < *nout; ++o)
+  {
+    uint64_t klen = 0;
+    char key[1025];
+    ok = fread(&klen, 8, 1, f) == 1 && klen < 1024;
+    ok = ok && fread(key, 1, klen, f) == klen;
+    if (!ok)
+      break;
+    key[klen] = 0;
+    ok = fn(o, key, f, ns);
+  }
+  fclose(f);
+  return ok ? DGX_OK : DGX_ERR_IO;
+}
+
+extern "C" {
+
+dgx_status dgx_read_dg_output_key(const char* path, uint64_t index, char* key_out,
+                                  uint64_t* nout)
+{
+  if (!path || !key_out || !nout)
+    return DGX_ERR_INVALID;
+  bool found = false;
+  dgx_status st = for_each_output(path, nullptr, nout, [&](uint64_t o, const char* key, FILE* f,
+                                                           uint64_t ns) {
+    if (o == index)
+    {
+      strcpy(key_out, key);
+      found = true;
+    }
+    return fseek(f, (long)(ns * sizeof(double)), SEEK_CUR) == 0;
+  });
+  if (st != DGX_OK)
+    return st;
+  return found ? DGX_OK : DGX_ERR_INVALID;
+}
+
+dgx_status dgx_read_dg_output(const char* path, const char* key, double* out)
+{
+  if (!path || !key || !out)
+    return DGX_ERR_INVALID;
+  uint64_t nout = 0;
+  bool found = false;
+  dgx_status st = for_each_output(path, nullptr, &nout, [&](uint64_t, const char* k, FILE* f,
+                                                            uint64_t ns) {
+    if (!found && strcmp(k, key) == 0)
+    {
+      found = true;
+      return fread(out, sizeof(double), ns, f) == ns;
+    }
+    return fseek(f, (long)(ns * sizeof(double)), SEEK_CUR) == 0;
+  });
+  if (st != DGX_OK)
+    return st;
+  return found ? DGX_OK : DGX_ERR_IO;
 }
 
 dgx_status dgx_read_dg(const char* path, double* nodes_out, double* U_out)

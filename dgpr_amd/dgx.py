@@ -118,6 +118,7 @@ def lib():
         L.dgx_residual_vjp.argtypes = [P_ctx, PD, PD]
         L.dgx_adjoint_rk4.argtypes = [P_ctx, D, U64, U64, PD, PD, PD]
         L.dgx_adjoint_release.argtypes = [P_ctx]
+        L.dgx_residual_vjp_device_noalloc.argtypes = [P_ctx, C.c_void_p, C.c_void_p]
         L.dgx_ensemble_create.argtypes = [C.c_int, C.POINTER(_CMesh), C.POINTER(Params),
                                           C.POINTER(C.c_void_p)]
         L.dgx_ensemble_destroy.argtypes = [C.c_void_p]
@@ -133,6 +134,11 @@ def lib():
         L.dgx_sample_output_device.argtypes = [P_ctx, C.c_int, C.c_int, C.c_void_p]
         L.dgx_update_mean.argtypes = [P_ctx, U64, U64]
         L.dgx_get_mean.argtypes = [P_ctx, PD]
+        L.dgx_set_mean.argtypes = [P_ctx, PD]
+        L.dgx_write_dg_outputs.argtypes = [C.c_char_p, U64, C.POINTER(_CMesh), U64,
+                                           C.POINTER(C.c_char_p), C.POINTER(PD)]
+        L.dgx_read_dg_output_key.argtypes = [C.c_char_p, U64, C.c_char_p, C.POINTER(U64)]
+        L.dgx_read_dg_output.argtypes = [C.c_char_p, C.c_char_p, PD]
         L.dgx_get_tuning.argtypes = [P_ctx, C.POINTER(C.c_int)]
         L.dgx_set_precision.argtypes = [P_ctx, C.c_int]
         L.dgx_get_precision.argtypes = [P_ctx, C.POINTER(C.c_int)]
@@ -149,10 +155,12 @@ def lib():
                    "dgx_get_tuning", "dgx_autotune", "dgx_set_precision", "dgx_get_precision",
                    "dgx_sample_output",
                    "dgx_sample_output_device", "dgx_update_mean", "dgx_get_mean",
+                   "dgx_set_mean", "dgx_write_dg_outputs", "dgx_read_dg_output_key",
+                   "dgx_read_dg_output",
                    "dgx_ensemble_create", "dgx_ensemble_destroy", "dgx_ensemble_member",
                    "dgx_ensemble_step", "dgx_ensemble_sync", "dgx_ensemble_get_stream",
                    "dgx_ensemble_autotune", "dgx_kinetic_energy", "dgx_residual_vjp",
-                   "dgx_adjoint_rk4", "dgx_adjoint_release"):
+                   "dgx_adjoint_rk4", "dgx_adjoint_release", "dgx_residual_vjp_device_noalloc"):
             getattr(L, nm).restype = st
         _lib = L
     return _lib
@@ -288,6 +296,34 @@ def read_dg(path):
     return int(ts.value), mesh, U
 
 
+def write_dg_outputs(path, tstep, mesh, outputs):
+    """write_state (io.cpp:33-89) of several named simstates, a dict
+    {key: state}; written in the reference's map order."""
+    keys = list(outputs)
+    arrs = [np.ascontiguousarray(outputs[k], dtype=np.float64).reshape(-1) for k in keys]
+    kk = (C.c_char_p * len(keys))(*[k.encode() for k in keys])
+    pp = (PD * len(keys))(*[_p(a) for a in arrs])
+    cm = mesh.cstruct()
+    _chk(lib().dgx_write_dg_outputs(path.encode(), tstep, C.byref(cm), len(keys), kk, pp))
+
+
+def read_dg_outputs(path):
+    """read_state (io.cpp:91-159) of every output: (tstep, mesh, {key: state})
+    with the keys in file order."""
+    ts, mesh, _ = read_dg(path)
+    nout = U64()
+    buf = C.create_string_buffer(1025)
+    _chk(lib().dgx_read_dg_output_key(path.encode(), 0, buf, C.byref(nout)))
+    out = {}
+    for i in range(int(nout.value)):
+        _chk(lib().dgx_read_dg_output_key(path.encode(), i, buf, C.byref(nout)))
+        key = buf.value.decode()
+        a = np.zeros(mesh.state_size)
+        _chk(lib().dgx_read_dg_output(path.encode(), key.encode(), _p(a)))
+        out[key] = a
+    return ts, mesh, out
+
+
 class Solver:
     """Device context: geometry, state and stage buffers on the current HIP
     device (``cuda_device_geometry`` + ``cuda_residual_workspace``)."""
@@ -412,6 +448,12 @@ class Solver:
         _chk(lib().dgx_residual_vjp(self.ctx, _p(w), _p(g)), self.ctx)
         return g
 
+    def residual_vjp_device_noalloc(self, w_ptr, g_ptr):
+        """dgx_residual_vjp_device_noalloc on device buffers given as integer
+        addresses: (df/dU)^T w into g with the library's fixed workspace."""
+        _chk(lib().dgx_residual_vjp_device_noalloc(self.ctx, C.c_void_p(int(w_ptr)),
+                                                   C.c_void_p(int(g_ptr))), self.ctx)
+
     def adjoint_rk4(self, dt, nsteps, interval=0, want_mu=True):
         """Final kinetic energy after nsteps RK4 steps and its gradients with
         respect to the initial state and to mu (discrete adjoint)."""
@@ -442,6 +484,10 @@ class Solver:
         M = np.zeros(self.nstate)
         _chk(lib().dgx_get_mean(self.ctx, _p(M)), self.ctx)
         return M
+
+    def set_mean(self, M):
+        M = np.ascontiguousarray(M, dtype=np.float64).reshape(-1)
+        _chk(lib().dgx_set_mean(self.ctx, _p(M)), self.ctx)
 
     @property
     def precision(self):

@@ -205,8 +205,13 @@ dgx_status dgx_sample_output(dgx_ctx* ctx, int var, int N, double* out_host);
 dgx_status dgx_update_mean(dgx_ctx* ctx, uint64_t time_step, uint64_t mean_start);
 
 /* The running mean in simstate layout (DGX_ERR_INVALID before the first
- * dgx_update_mean). */
+ * dgx_update_mean or dgx_set_mean). */
 dgx_status dgx_get_mean(dgx_ctx* ctx, double* mean_host);
+
+/* Sets the running mean from a simstate, e.g. the "mean" output of a .dg
+ * file that a restarted run continues to average into (the CPU driver keeps
+ * the mean it read, dg.cpp:44-72, 122-131). */
+dgx_status dgx_set_mean(dgx_ctx* ctx, const double* mean_host);
 
 /* ---- batched ensembles (no counterpart in the reference, which runs one
  * case per process, dg.cpp / cudg.cu) ----------------------------------------
@@ -249,6 +254,19 @@ dgx_status dgx_kinetic_energy(dgx_ctx* ctx, double* J);
 /* g = (df/dU)^T w at the context's state: the vector-Jacobian product of
  * residual()'s f = -Minv R (solve.cpp:150-807); w and g in simstate layout. */
 dgx_status dgx_residual_vjp(dgx_ctx* ctx, const double* w_host, double* g_host);
+
+/* Allocation-free form of dgx_residual_vjp on device buffers (no
+ * counterpart in the reference, which has no adjoint): for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * optimisation loops, harnesses that forbid hipMalloc).  w_dev and g_dev are
+ * device buffers in simstate layout; the workspace (3 states + 2 trace-record
+ * sets) comes from a fixed 64 MiB device array that the library holds on
+ * every device it runs on, so meshes needing more are rejected with
+ * DGX_ERR_INVALID (at p = 3 about 3300 elements fit, at p = 2 about 6600).
+ * Calls on one device share that array: calls that overlap on one device
+ * (other threads, other contexts) race and must be serialised by the
+ * caller.  Returns after completion; results equal dgx_residual_vjp's. */
+dgx_status dgx_residual_vjp_device_noalloc(dgx_ctx* ctx, const double* w_dev, double* g_dev);
 
 /* Runs nsteps classic RK4 steps of size dt from the context's state (which
  * ends at the final state, as after dgx_step) and returns the final kinetic
@@ -373,7 +391,7 @@ dgx_status dgx_turbulent_state(const dgx_mesh* mesh, const dgx_params* params,
                                double l, double* U_out);
 
 /* .dg checkpoint file (io.cpp:33-159): write_state with one output named
- * "state", and read_state of that output.  dgx_read_dg_header fills
+ * "state", and read_state of that output (of the first output so named).  dgx_read_dg_header fills
  * tstep and a mesh header (nodes NULL); dgx_read_dg then reads nodes and
  * state into caller buffers sized from that header. */
 dgx_status dgx_write_dg(const char* path, uint64_t tstep, const dgx_mesh* mesh,
@@ -381,6 +399,26 @@ dgx_status dgx_write_dg(const char* path, uint64_t tstep, const dgx_mesh* mesh,
 dgx_status dgx_read_dg_header(const char* path, uint64_t* tstep,
                               dgx_mesh* mesh);
 dgx_status dgx_read_dg(const char* path, double* nodes_out, double* U_out);
+
+/* write_state with several named outputs (io.cpp:33-89), e.g. "state" and
+ * the running "mean" of the CPU driver (dg.cpp:122-131): keys[o] names the
+ * simstate states[o]; keys are unique, 1..1023 characters.  Outputs are
+ * written in the order the reference's map<simstate> visits them (djb2
+ * bucket of the key, data_structures.cpp:537-673), so the file is byte for
+ * byte the reference's ("state" before "mean"). */
+dgx_status dgx_write_dg_outputs(const char* path, uint64_t tstep, const dgx_mesh* mesh,
+                                uint64_t nout, const char* const* keys,
+                                const double* const* states);
+
+/* read_state of every output (io.cpp:91-159): *nout receives the file's
+ * output count and key_out (1024 bytes) the key of output number index in
+ * file order (DGX_ERR_INVALID when index >= *nout). */
+dgx_status dgx_read_dg_output_key(const char* path, uint64_t index, char* key_out,
+                                  uint64_t* nout);
+
+/* The simstate of the output named key (DGX_ERR_IO when absent); out holds
+ * nelem*5*(p+1)^3 doubles sized from dgx_read_dg_header. */
+dgx_status dgx_read_dg_output(const char* path, const char* key, double* out);
 
 #ifdef __cplusplus
 }

@@ -341,6 +341,68 @@ inline void write_state(const std::string& path, u64 tstep, const core_geometry&
   check(dgx_write_dg(path.c_str(), tstep, &m, U.U.data()));
 }
 
+// map<simstate> of the drivers (dg.cpp:44, cudg.cu:44): the named outputs a
+// .dg file carries ("state", and "mean" once the CPU driver averages)
+struct outputs_map
+{
+  std::vector<std::pair<std::string, simstate>> items;
+
+  bool has(const std::string& key) const
+  {
+    for (const auto& it : items)
+      if (it.first == key)
+        return true;
+    return false;
+  }
+  // map::add: replaces an existing output of that key
+  simstate& add(const std::string& key, const simstate& s)
+  {
+    for (auto& it : items)
+      if (it.first == key)
+        return it.second = s;
+    items.emplace_back(key, s);
+    return items.back().second;
+  }
+  simstate& get(const std::string& key)
+  {
+    for (auto& it : items)
+      if (it.first == key)
+        return it.second;
+    throw error(DGX_ERR_INVALID, "no output named \"" + key + "\"");
+  }
+};
+
+// read_state / write_state (io.cpp:33-159) of every output
+inline void read_state(const std::string& path, u64& tstep, core_geometry& g, outputs_map& out)
+{
+  simstate U;
+  read_state(path, tstep, g, U);
+  out.items.clear();
+  u64 nout = 0;
+  char key[1025];
+  check(dgx_read_dg_output_key(path.c_str(), 0, key, &nout));
+  for (u64 i = 0; i < nout; ++i)
+  {
+    check(dgx_read_dg_output_key(path.c_str(), i, key, &nout));
+    simstate& s = out.add(key, simstate(g));
+    check(dgx_read_dg_output(path.c_str(), key, s.U.data()));
+  }
+}
+
+inline void write_state(const std::string& path, u64 tstep, const core_geometry& g,
+                        const outputs_map& out)
+{
+  dgx_mesh m = g.mesh();
+  std::vector<const char*> keys;
+  std::vector<const double*> states;
+  for (const auto& it : out.items)
+  {
+    keys.push_back(it.first.c_str());
+    states.push_back(it.second.U.data());
+  }
+  check(dgx_write_dg_outputs(path.c_str(), tstep, &m, keys.size(), keys.data(), states.data()));
+}
+
 // cuda_device_geometry + cuda_residual_workspace: the mesh, the state and
 // the stage buffers on the current HIP device
 class device_geometry
@@ -363,6 +425,14 @@ class device_geometry
   void upload(const simstate& U) { check(dgx_set_state(ctx_, U.U.data()), ctx_); }
   // D2H + unshuffle_state (cudg.cu:131-136)
   void download(simstate& U) const { check(dgx_get_state(ctx_, U.U.data()), ctx_); }
+  // update_mean (output.cpp:39-49) of the device state, kept on the device;
+  // restarts from zero when tstep == mean_start
+  void update_mean(u64 tstep, u64 mean_start)
+  {
+    check(dgx_update_mean(ctx_, tstep, mean_start), ctx_);
+  }
+  void upload_mean(const simstate& M) { check(dgx_set_mean(ctx_, M.U.data()), ctx_); }
+  void download_mean(simstate& M) const { check(dgx_get_mean(ctx_, M.U.data()), ctx_); }
   // one step; norm receives the RMS residual norm of the first stage
   // (NaN when the step went unstable, reported as DGX_ERR_NAN)
   dgx_status step(int scheme, double dt, double& norm)

@@ -147,3 +147,43 @@ def test_driver_reports_instability(driver, tmp_path):
     last = max(norms)
     assert np.isnan(norms[last]) and last < 199
     assert not os.path.exists(out + ".dg")
+
+
+def test_driver_running_mean_and_restart(orc, driver, tmp_path):
+    """-mean_start 2: from step 2 on the driver keeps the running mean of
+    the state (update_mean after each step, output.cpp:39-49, dg.cpp:122-131)
+    and writes it as the "mean" output next to "state"; the oracle's
+    update_mean over its own states gives the same mean.  A restart from
+    the checkpoint before step 4 reads the mean and continues it."""
+    c = CASE
+    out = str(tmp_path / "avg")
+    r, _ = run(driver, case_args(c) + ["-niter", "6", "-chkint", "4", "-mean_start", "2",
+                                       "-ofile", out])
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, states, _ = oracle_reference(orc, c, 6)
+    mean = np.zeros_like(states[0])
+    means = {}
+    for t in range(6):  # step t produces states[t + 1]
+        if t >= 2:
+            mean = orc.update_mean(states[t + 1], mean, t, 2)
+        means[t] = mean.copy()
+    ts, _, outs = dgx.read_dg_outputs(out + ".dg")
+    assert ts == 5 and list(outs) == ["state", "mean"]
+    e = rel(outs["mean"], means[5])
+    print("  final mean rel err %.3e" % e, flush=True)
+    assert e < TOL and rel(outs["state"], states[6]) < TOL
+    # the checkpoint before step 3 holds the mean after steps 2 only
+    ts, _, outs3 = dgx.read_dg_outputs(out + "3.dg")
+    assert ts == 3 and rel(outs3["mean"], means[2]) < TOL
+    # restart from it: the run resumes at step 4 (the step after the one the
+    # file holds, cudg.cu:67) on the state after three steps, and steps 4
+    # and 5 continue the mean read from the file with their own step numbers
+    out2 = str(tmp_path / "avg2")
+    r, _ = run(driver, ["-ifile", out + "3", "-niter", "2", "-dt", repr(c["dt"]),
+                        "-scheme", "RK4", "-mean_start", "2", "-ofile", out2])
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = orc.update_mean(states[4], means[2], 4, 2)
+    m = orc.update_mean(states[5], m, 5, 2)
+    ts, _, outs2 = dgx.read_dg_outputs(out2 + ".dg")
+    assert ts == 5 and list(outs2) == ["state", "mean"]
+    assert rel(outs2["state"], states[5]) < TOL and rel(outs2["mean"], m) < TOL

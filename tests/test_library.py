@@ -195,3 +195,38 @@ def test_affine_classifier_scales_with_element_size():
     assert not dgx.Mesh(2, 1, n, n, n, [dgx.PERIODIC] * 6, twisted.reshape(-1)).affine
     q2 = dgx.box_nodes(2, n, n, n, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 0.0, 0)
     assert not dgx.Mesh(2, 2, n, n, n, [dgx.PERIODIC] * 6, q2).affine
+
+
+def test_dg_file_with_several_outputs(tmp_path):
+    """write_state of the map {"state", "mean"} (io.cpp:33-89): outputs in the
+    order the reference's map visits them, djb2(key) % 256 bucket order
+    (data_structures.cpp:537-673: "state" in bucket 6 before "mean" in 102),
+    whatever order they are given in; read back by key and in file order;
+    the single-output reader still finds "state"."""
+    def bucket(k):
+        h = 5381
+        for ch in k.encode():
+            h = (h * 33 + ch) & 0xFFFFFFFF
+        return h % 256
+
+    assert bucket("state") == 6 and bucket("mean") == 102
+    c = cases.tgv_case(n=2, p=1, seed=5)
+    M = np.random.default_rng(1).uniform(size=c.mesh.state_size)
+    path = str(tmp_path / "two.dg")
+    dgx.write_dg_outputs(path, 9, c.mesh, {"mean": M, "state": c.U0})
+    raw = open(path, "rb").read()
+    ne, nn, ns = 8, 8 * 8 * 3, 8 * 5 * 8
+    assert struct.unpack("<Q", raw[96:104])[0] == 2
+    o = 104 + 8 * nn
+    assert raw[o:o + 13] == struct.pack("<Q", 5) + b"state"
+    assert np.array_equal(np.frombuffer(raw[o + 13:o + 13 + 8 * ns], "<f8"), c.U0)
+    o += 13 + 8 * ns
+    assert raw[o:o + 12] == struct.pack("<Q", 4) + b"mean"
+    assert np.array_equal(np.frombuffer(raw[o + 12:], "<f8"), M)
+    ts, mesh, outs = dgx.read_dg_outputs(path)
+    assert ts == 9 and list(outs) == ["state", "mean"] and ne == mesh.nelem
+    assert np.array_equal(outs["mean"], M) and np.array_equal(outs["state"], c.U0)
+    _, _, U = dgx.read_dg(path)
+    assert np.array_equal(U, c.U0)
+    with pytest.raises(dgx.DgxError):
+        dgx.write_dg_outputs(path, 0, c.mesh, {"state": c.U0, "": M})
