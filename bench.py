@@ -403,7 +403,10 @@ def run_mixed(a, h, s, case, value64, stream, ev0, ev1, norms_dev):
     s.precision = "mixed"
     s.step_async(case.dt, case.scheme, a.warmup)
     s.sync()
-    waves = s.autotune(reps=3)[0] if a.autotune else s.stage_waves
+    if a.autotune:
+        tile, waves, _ = s.autotune_tiles(reps=3)
+    else:
+        tile, waves = s.tile, s.stage_waves
     s.set_state(case.U0)
     gpu_s = timed_steps(h, s, case, a.steps, stream, ev0, ev1, norms_dev)
     value = case.dof * case.stages * a.steps / gpu_s / 1e9
@@ -425,6 +428,7 @@ def run_mixed(a, h, s, case, value64, stream, ev0, ev1, norms_dev):
         "speedup_vs_fp64": value / value64,
         "ms_per_step": 1e3 * gpu_s / a.steps,
         "avg_launch_us": gpu_s_nn / (a.steps * case.stages) * 1e6,
+        "stage_tile": tile,
         "stage_waves_per_simd": waves,
         "form": s.MIXED_FORM,
         "error_vs_fp64": field_errors(Um, U64, case, nerr, k64, eps64, kmx, epsmx),
@@ -468,11 +472,12 @@ def main():
     stream = C.c_void_p(s.stream())
     s.step_async(case.dt, case.scheme, a.warmup)
     s.sync()
-    # launch tuning (untimed): each compiled occupancy target on this mesh
+    # launch tuning (untimed): each compiled (tile, occupancy target) pair
+    # of the stage kernel on this mesh
     if a.autotune:
-        waves, tune_us = s.autotune(reps=3)
+        tile, waves, tune_us = s.autotune_tiles(reps=3)
     else:
-        waves, tune_us = s.stage_waves, None
+        tile, waves, tune_us = s.tile, s.stage_waves, None
     norms_dev = C.c_void_p()
     h.hipMalloc(C.byref(norms_dev), 8 * a.steps)
     ev0, ev1 = C.c_void_p(), C.c_void_p()
@@ -507,13 +512,14 @@ def main():
         h.hipFree(src)
         h.hipFree(dst)
 
+    kernel_name = s.stage_kernel_name(waves, tile)
     mixed = None if a.no_mixed else run_mixed(a, h, s, case, value, stream, ev0, ev1, norms_dev)
     h.hipFree(norms_dev)
     s.close()
 
     # HBM bytes and executed FP64 flops per stage-kernel launch, from the
     # committed rocprofv3 counter summary of the same kernel and workload
-    kname = s.stage_kernel_name(waves)
+    kname = kernel_name
     traffic = flops = None
     cf = os.path.join(ROOT, "profiles", "counters_%s.json" % a.workload)
     if os.path.exists(cf):
@@ -550,8 +556,10 @@ def main():
             "timed": "%d steps from the initial state, each with the RMS(R) of its first "
                      "stage kept on the device" % a.steps,
             "parallelism": "single GPU",
+            "stage_tile": tile,
             "stage_waves_per_simd": waves,
-            "autotune_us": None if tune_us is None else [round(float(x), 2) for x in tune_us],
+            "autotune_us": None if tune_us is None else
+            {"tile%d" % t: [round(float(x), 2) for x in tune_us[t]] for t in range(2)},
         },
         "without_norm": {
             "value": case.dof * case.stages * a.steps / gpu_s_nn / 1e9,

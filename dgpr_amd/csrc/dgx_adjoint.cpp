@@ -117,7 +117,7 @@ hipError_t stage(dgx_ctx* c, const Phys& ph, const double* Vin, const double* Ti
   sa.accmode = 0;
   sa.trace32 = 0;
   return rhs_dispatch(c->p, c->q, mesh_args(c), ph, sa, c->lam, c->status, c->stage_waves,
-                      c->stream);
+                      c->stream, c->stage_tile);
 }
 
 // one RK4 step of the context's state (the dgx_step path)
@@ -131,7 +131,7 @@ hipError_t step_ctx(dgx_ctx* c, double dt)
   int ns = stage_plan(c, DGX_RK4, dt, sa);
   for (int k = 0; k < ns && e == hipSuccess; ++k)
     e = rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa[k], c->lam, c->status,
-                     c->stage_waves, c->stream);
+                     c->stage_waves, c->stream, c->stage_tile);
   if (e == hipSuccess)
     c->tu_valid = true;
   return e;
@@ -427,7 +427,7 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
         sa.accmode = accmode[i];
         sa.trace32 = 0;
         AHIP(c, rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status,
-                             c->stage_waves, c->stream));
+                             c->stage_waves, c->stream, c->stage_tile));
       }
     AHIP(c, copy_state(c, c->U, final_state.p));
     c->tu_valid = false;

@@ -271,6 +271,7 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
       return bail("hipMalloc(traces)", e);
   c->tu_valid = false;
   c->stage_waves = default_stage_waves(c->p, c->q);
+  c->stage_tile  = 0;
   c->prec = DGX_PREC_FP64;
   c->bytes = 5 * sb + 3 * tb + (nn + nlam + (size_t)c->npartial) * sizeof(double) + 16;
   if ((e = hipMemcpyAsync(c->nodes, mesh->nodes, nn * sizeof(double), hipMemcpyHostToDevice,
@@ -410,7 +411,7 @@ static hipError_t rhs(dgx_ctx* c, const double* Vin, const double* Tin, const do
   sa.accmode = accmode;
   sa.trace32 = c->prec == DGX_PREC_MIXED;
   return rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status, c->stage_waves,
-                      c->stream);
+                      c->stream, c->stage_tile);
 }
 
 }  // extern "C"
@@ -529,7 +530,7 @@ static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm,
     if (k == 0 && want_norm && scheme != DGX_RK4)
       sa[0].fout = c->fs;
     if ((e = rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa[k], c->lam, c->status,
-                          c->stage_waves, c->stream)) != hipSuccess)
+                          c->stage_waves, c->stream, c->stage_tile)) != hipSuccess)
       return e;
     if (k == 0 && want_norm)
     {
@@ -622,7 +623,7 @@ dgx_status dgx_get_precision(const dgx_ctx* c, int* precision)
 
 dgx_status dgx_set_tuning(dgx_ctx* c, int stage_waves)
 {
-  if (!c || stage_waves < 1 || stage_waves > 3)
+  if (!c || stage_waves < 1 || stage_waves > 3 || !tile_valid(c->p, c->q, c->stage_tile, stage_waves))
     return DGX_ERR_INVALID;
   c->stage_waves = stage_waves;
   return DGX_OK;
@@ -636,8 +637,29 @@ dgx_status dgx_get_tuning(const dgx_ctx* c, int* stage_waves)
   return DGX_OK;
 }
 
-// Times the stage kernel at each occupancy target on the context's own mesh:
-// U (with its traces) -> Ub, TUb, which are stage scratch between steps.
+dgx_status dgx_set_tile(dgx_ctx* c, int tile)
+{
+  if (!c || tile < 0 || tile > 1)
+    return DGX_ERR_INVALID;
+  if (!tile_valid(c->p, c->q, tile, tile ? 2 : 1))
+    return fail(c, DGX_ERR_UNSUPPORTED, "no tiled stage kernel for p = %d, q = %d", c->p, c->q);
+  c->stage_tile = tile;
+  if (!tile_valid(c->p, c->q, tile, c->stage_waves))
+    c->stage_waves = 2;  // the tiled kernels are compiled for 2 and 3 waves per SIMD
+  return DGX_OK;
+}
+
+dgx_status dgx_get_tile(const dgx_ctx* c, int* tile)
+{
+  if (!c || !tile)
+    return DGX_ERR_INVALID;
+  *tile = c->stage_tile;
+  return DGX_OK;
+}
+
+// Times the stage kernel at each occupancy target of the current tile on
+// the context's own mesh: U (with its traces) -> Ub, TUb, which are stage
+// scratch between steps.  Targets the tile does not compile read 0.
 dgx_status dgx_autotune(dgx_ctx* c, int reps, int* chosen, double us_out[3])
 {
   if (!c || reps < 1)
@@ -648,9 +670,16 @@ dgx_status dgx_autotune(dgx_ctx* c, int reps, int* chosen, double us_out[3])
   HIPCHK(c, hipEventCreate(&e1));
   int keep = c->stage_waves, best = keep;
   double best_us = 0.0;
+  bool have = false;
   dgx_status st = DGX_OK;
   for (int w = 1; w <= 3 && st == DGX_OK; ++w)
   {
+    if (!tile_valid(c->p, c->q, c->stage_tile, w))
+    {
+      if (us_out)
+        us_out[w - 1] = 0.0;
+      continue;
+    }
     c->stage_waves = w;
     hipError_t e = rhs(c, c->U, c->TU, c->U, nullptr, c->Ub, c->TUb, nullptr, 1.0, 0.0, 0.0, 0);
     if (e == hipSuccess)
@@ -672,10 +701,11 @@ dgx_status dgx_autotune(dgx_ctx* c, int reps, int* chosen, double us_out[3])
     double us = 1e3 * (double)ms / reps;
     if (us_out)
       us_out[w - 1] = us;
-    if (w == 1 || us < best_us)
+    if (!have || us < best_us)
     {
       best_us = us;
       best = w;
+      have = true;
     }
   }
   (void)hipEventDestroy(e0);
@@ -686,6 +716,53 @@ dgx_status dgx_autotune(dgx_ctx* c, int reps, int* chosen, double us_out[3])
   if (st != DGX_OK)
     return st;
   return check_status(c);
+}
+
+dgx_status dgx_autotune_tiles(dgx_ctx* c, int reps, int* tile_out, int* waves_out,
+                              double us_out[6])
+{
+  if (!c || reps < 1)
+    return DGX_ERR_INVALID;
+  const int keep_tile = c->stage_tile, keep_waves = c->stage_waves;
+  int best_tile = keep_tile, best_waves = keep_waves;
+  double best_us = 0.0;
+  bool have = false;
+  for (int t = 0; t <= 1; ++t)
+  {
+    double us[3] = {0.0, 0.0, 0.0};
+    if (tile_valid(c->p, c->q, t, 2))
+    {
+      c->stage_tile = t;
+      if (!tile_valid(c->p, c->q, t, c->stage_waves))
+        c->stage_waves = 2;
+      int w = 0;
+      dgx_status st = dgx_autotune(c, reps, &w, us);
+      if (st != DGX_OK)
+      {
+        c->stage_tile  = keep_tile;
+        c->stage_waves = keep_waves;
+        return st;
+      }
+      for (int k = 0; k < 3; ++k)
+        if (us[k] > 0.0 && (!have || us[k] < best_us))
+        {
+          best_us    = us[k];
+          best_tile  = t;
+          best_waves = k + 1;
+          have       = true;
+        }
+    }
+    if (us_out)
+      for (int k = 0; k < 3; ++k)
+        us_out[3 * t + k] = us[k];
+  }
+  c->stage_tile  = best_tile;
+  c->stage_waves = best_waves;
+  if (tile_out)
+    *tile_out = best_tile;
+  if (waves_out)
+    *waves_out = best_waves;
+  return DGX_OK;
 }
 
 dgx_status dgx_sample_output_device(dgx_ctx* c, int var, int N, double* out_dev)

@@ -33,6 +33,7 @@ struct dgx_ctx
   double* TUb;
   bool tu_valid;    // TU holds the traces of the current U
   int stage_waves;  // occupancy target of the stage kernel (1..3)
+  int stage_tile;   // 0: one-row stage kernel, 1: tiled stage kernel
   int prec;         // DGX_PREC_FP64 or DGX_PREC_MIXED (float trace records)
   int affine;       // q = 1 and every element a parallelepiped
   double* mean;     // running mean (Gauss-nodal), allocated on first use

@@ -838,6 +838,8 @@ void dgx_rhs_batch_kernel(const BatchItem* __restrict__ items, int nitems, int* 
   stage_body<P, Q, double, AFF>(it.mesh, it.ph, it.sa, it.lam, status, b - it.block0);
 }
 
+#include "dgx_tile.hpp"
+
 // BR2 lifting coefficient of each element's side of each of its faces:
 //   lam[e][f][pt] = fJ(pt) * sum_m L_hi[m]^2 / (w_m detJ(line node m))
 // (the pointwise form of Phi_f M_e^-1 Phi_f^T W_f; geometry only).
@@ -1236,10 +1238,41 @@ __global__ __launch_bounds__(256) void dgx_mean_kernel(const double* __restrict_
 
 // ---- launch helpers (host) -----------------------------------------------
 
+// rows per tile of the tiled stage kernel at each order (0: none compiled):
+// p = 3: two one-element rows, so each face subpass is the 64 points of one
+// direction of two elements; p = 1: one row of 8 elements (16 faces of 4
+// points per direction); p = 2: three rows of 2 elements (12 faces of 9
+// points per direction in two subpasses)
+constexpr int tile_rows(int p)
+{
+  return p == 3 ? 2 : (p == 2 ? 3 : (p == 1 ? 1 : 0));
+}
+
+template <int P, int Q, typename TT, bool AFF>
+hipError_t launch_tile_t(const MeshArgs& m, const Phys& ph, const StageArgs& sa,
+                         const double* lam, int* status, int waves, hipStream_t st)
+{
+  constexpr int R = tile_rows(P);
+  constexpr int E = TileDims<P, R>::E;
+  int nblk = (m.nelem + E - 1) / E;
+  if (waves == 2)
+    hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 2, TT, AFF>), dim3(nblk), dim3(64), 0, st,
+                       m, ph, sa, lam, status);
+  else if (waves == 3)
+    hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 3, TT, AFF>), dim3(nblk), dim3(64), 0, st,
+                       m, ph, sa, lam, status);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 template <int P, int Q, typename TT, bool AFF>
 hipError_t launch_rhs_t(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
-                        int* status, int waves, hipStream_t st)
+                        int* status, int waves, int tile, hipStream_t st)
 {
+  if constexpr (Q == 1)
+    if (tile)
+      return launch_tile_t<P, Q, TT, AFF>(m, ph, sa, lam, status, waves, st);
   constexpr int EPW = Dims<P, Q>::EPW;
   int nblk = (m.nelem + EPW - 1) / EPW;
   if (waves == 1)
@@ -1259,21 +1292,21 @@ hipError_t launch_rhs_t(const MeshArgs& m, const Phys& ph, const StageArgs& sa, 
 // affine q = 1 meshes take the constant-Jacobian instantiation
 template <int P, int Q, typename TT>
 hipError_t launch_rhs_a(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
-                        int* status, int waves, hipStream_t st)
+                        int* status, int waves, int tile, hipStream_t st)
 {
   if constexpr (Q == 1)
     if (m.affine)
-      return launch_rhs_t<P, Q, TT, true>(m, ph, sa, lam, status, waves, st);
-  return launch_rhs_t<P, Q, TT, false>(m, ph, sa, lam, status, waves, st);
+      return launch_rhs_t<P, Q, TT, true>(m, ph, sa, lam, status, waves, tile, st);
+  return launch_rhs_t<P, Q, TT, false>(m, ph, sa, lam, status, waves, tile, st);
 }
 
 template <int P, int Q>
 hipError_t launch_rhs(const MeshArgs& m, const Phys& ph, const StageArgs& sa, const double* lam,
-                      int* status, int waves, hipStream_t st)
+                      int* status, int waves, int tile, hipStream_t st)
 {
   if (sa.trace32)
-    return launch_rhs_a<P, Q, float>(m, ph, sa, lam, status, waves, st);
-  return launch_rhs_a<P, Q, double>(m, ph, sa, lam, status, waves, st);
+    return launch_rhs_a<P, Q, float>(m, ph, sa, lam, status, waves, tile, st);
+  return launch_rhs_a<P, Q, double>(m, ph, sa, lam, status, waves, tile, st);
 }
 
 template <int P, int Q, bool AFF>
@@ -1385,10 +1418,19 @@ int resid_out_blocks(int p, int nelem)
   }                                                                                         \
   return hipErrorInvalidValue;
 
-hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
-                        const double* lam, int* status, int waves, hipStream_t st)
+// the tiled kernels are compiled for q = 1 (with higher-order geometry the
+// extra rows of accumulators spill)
+bool tile_valid(int p, int q, int tile, int waves)
 {
-#define CALL_RHS(P, Q) launch_rhs<P, Q>(m, ph, sa, lam, status, waves, st)
+  return tile == 0 || (tile == 1 && q == 1 && tile_rows(p) > 0 && (waves == 2 || waves == 3));
+}
+
+hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
+                        const double* lam, int* status, int waves, hipStream_t st, int tile)
+{
+  if (!tile_valid(p, q, tile, waves))
+    return hipErrorInvalidValue;
+#define CALL_RHS(P, Q) launch_rhs<P, Q>(m, ph, sa, lam, status, waves, tile, st)
   DGX_DISPATCH_PQ(p, q, CALL_RHS)
 #undef CALL_RHS
 }

@@ -90,8 +90,12 @@ struct BatchItem
 
 // waves: occupancy target of the compiled stage kernel (1, 2 or 3 waves per
 // SIMD), chosen by default_stage_waves or by timing (dgx_autotune)
+// tile 0: the one-wavefront-per-row kernel (faces of all directions mixed in
+// each pass); tile 1: the tiled kernel (rows per tile fixed per order, faces
+// one direction at a time), compiled for 2 and 3 waves per SIMD
 hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
-                        const double* lam, int* status, int waves, hipStream_t st);
+                        const double* lam, int* status, int waves, hipStream_t st, int tile = 0);
+bool tile_valid(int p, int q, int tile, int waves);
 // one stage of several meshes of the same (p, q): items_dev (device memory,
 // sorted by block0) covers nblocks blocks in total; affine: every member is
 // an affine q = 1 mesh (MeshArgs::affine)

@@ -143,6 +143,10 @@ def lib():
         L.dgx_set_precision.argtypes = [P_ctx, C.c_int]
         L.dgx_get_precision.argtypes = [P_ctx, C.POINTER(C.c_int)]
         L.dgx_autotune.argtypes = [P_ctx, C.c_int, C.POINTER(C.c_int), PD]
+        L.dgx_set_tile.argtypes = [P_ctx, C.c_int]
+        L.dgx_get_tile.argtypes = [P_ctx, C.POINTER(C.c_int)]
+        L.dgx_autotune_tiles.argtypes = [P_ctx, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), PD]
         for nm in ("dgx_create", "dgx_destroy", "dgx_set_params", "dgx_set_state",
                    "dgx_get_state", "dgx_set_state_device", "dgx_get_state_device",
                    "dgx_residual", "dgx_residual_device", "dgx_step", "dgx_step_async",
@@ -153,6 +157,7 @@ def lib():
                    "dgx_write_dg", "dgx_read_dg_header", "dgx_read_dg", "dgx_set_tuning",
                    "dgx_mesh_is_affine",
                    "dgx_get_tuning", "dgx_autotune", "dgx_set_precision", "dgx_get_precision",
+                   "dgx_set_tile", "dgx_get_tile", "dgx_autotune_tiles",
                    "dgx_sample_output",
                    "dgx_sample_output_device", "dgx_update_mean", "dgx_get_mean",
                    "dgx_set_mean", "dgx_write_dg_outputs", "dgx_read_dg_output_key",
@@ -503,10 +508,38 @@ class Solver:
     # what DGX_PREC_MIXED computes in FP32 (the rest stays FP64)
     MIXED_FORM = "FP32 face trace records; geometry, fluxes, Roe, inverse mass, RK in FP64"
 
-    def stage_kernel_name(self, waves=None):
-        """rocprof name prefix of the stage kernel this context launches."""
+    # rows per tile of the tiled stage kernel at p = 1, 2, 3
+    TILE_ROWS = {1: 1, 2: 3, 3: 2}
+
+    def stage_kernel_name(self, waves=None, tile=None):
+        """Name (template arguments P, Q[, rows], W) of the stage kernel this
+        context launches."""
         w = self.stage_waves if waves is None else waves
+        t = self.tile if tile is None else tile
+        if t:
+            return "dgx_tile_stage_kernel<%d,%d,%d,%d>" % (self.mesh.p, self.mesh.q,
+                                                           self.TILE_ROWS[self.mesh.p], w)
         return "dgx_rhs_stage_kernel<%d,%d,%d>" % (self.mesh.p, self.mesh.q, w)
+
+    @property
+    def tile(self):
+        """Stage tile: 0 (one row of elements per wavefront, mixed-direction
+        face passes) or 1 (tiled, one direction per face subpass)."""
+        t = C.c_int()
+        _chk(lib().dgx_get_tile(self.ctx, C.byref(t)), self.ctx)
+        return t.value
+
+    @tile.setter
+    def tile(self, t):
+        _chk(lib().dgx_set_tile(self.ctx, int(t)), self.ctx)
+
+    def autotune_tiles(self, reps=3):
+        """Times every (tile, waves) pair, keeps the fastest; returns
+        (tile, waves, us[2][3]) with 0 where a pair is not compiled."""
+        t, w = C.c_int(), C.c_int()
+        us = np.zeros(6)
+        _chk(lib().dgx_autotune_tiles(self.ctx, reps, C.byref(t), C.byref(w), _p(us)), self.ctx)
+        return t.value, w.value, us.reshape(2, 3)
 
     def autotune(self, reps=5):
         """Times each compiled occupancy target on this mesh, keeps the

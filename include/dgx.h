@@ -319,11 +319,30 @@ dgx_status dgx_get_precision(const dgx_ctx* ctx, int* precision);
 dgx_status dgx_set_tuning(dgx_ctx* ctx, int stage_waves);
 dgx_status dgx_get_tuning(const dgx_ctx* ctx, int* stage_waves);
 
-/* Times every compiled occupancy target on this context's mesh and state
- * (reps launches each into scratch buffers; the state is unchanged), keeps
- * the fastest and returns it in *chosen (may be NULL).  us_out (may be NULL)
- * receives the three average launch times in microseconds. */
+/* Times every compiled occupancy target of the current stage tile on this
+ * context's mesh and state (reps launches each into scratch buffers; the
+ * state is unchanged), keeps the fastest and returns it in *chosen (may be
+ * NULL).  us_out (may be NULL) receives the average launch times in
+ * microseconds at 1, 2 and 3 waves per SIMD (0 for a target the tile does
+ * not compile). */
 dgx_status dgx_autotune(dgx_ctx* ctx, int reps, int* chosen, double us_out[3]);
+
+/* Stage tile: 0 runs the stage kernel that maps one row of whole elements
+ * (one element at p = 3, 2 at p = 2, 8 at p = 1) to a wavefront and mixes
+ * the faces of all directions in each face pass; 1 runs the tiled kernel:
+ * several rows per wavefront (2 at p = 3, 3 at p = 2, 1 at p = 1), each lane
+ * owning one node per row, faces processed one reference direction at a
+ * time in whole-face subpasses (compiled for q = 1 meshes at 2 and 3 waves
+ * per SIMD; selecting it moves a target of 1 to 2; DGX_ERR_UNSUPPORTED for
+ * q > 1).  Results agree to rounding. */
+dgx_status dgx_set_tile(dgx_ctx* ctx, int tile);
+dgx_status dgx_get_tile(const dgx_ctx* ctx, int* tile);
+
+/* Times every (tile, occupancy target) pair on this context's mesh as
+ * dgx_autotune does and keeps the fastest; us_out[3 * tile + waves - 1]
+ * (may be NULL) receives the times (0 where not compiled). */
+dgx_status dgx_autotune_tiles(dgx_ctx* ctx, int reps, int* tile_out, int* waves_out,
+                              double us_out[6]);
 
 /* ---- host-side helpers (no device work) --------------------------------- */
 

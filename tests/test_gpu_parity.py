@@ -232,3 +232,67 @@ def test_inflow_without_mach_root_is_reported(orc, built):
         geo = orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
         _, f_o = geo.residual(orc_params(orc, case.params), case.U0)
         assert rel(f, f_o) < RES_TOL
+
+
+# the q = 1 cases (the tiled kernels are compiled for q = 1)
+TILE_CASES = ["bump_freestream_p2q1", "channel_inviscid_p2", "channel_p1", "channel_p2",
+              "channel_p3", "ragged_1x2x1_p3", "ragged_2x1x3_p1", "single_element_p2",
+              "single_element_p3", "tgv3_p3", "tgv4_p1", "tgv4_p2", "tgv_curved_p2",
+              "tgv_curved_p3", "tgv_near_affine_p3"]
+
+
+@pytest.mark.parametrize("name", TILE_CASES)
+def test_tiled_kernel_parity(name, orc, built):
+    """The tiled stage kernel (several rows of elements per wavefront, faces
+    one direction at a time) on every q = 1 case: the residual and the
+    state after N steps match the oracle as the one-row kernel's do, and the
+    two kernels agree with each other to rounding."""
+    case = CASES[name]()
+    m = case.mesh
+    assert m.q == 1
+    print("case", name, "p", m.p, flush=True)
+    geo = orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
+    prm = orc_params(orc, case.params)
+    R_o, f_o = geo.residual(prm, case.U0)
+    U_o, n_o = geo.step(prm, case.U0, case.dt, case.scheme, case.nsteps)
+    with dgx.Solver(m, case.params) as s:
+        s.set_state(case.U0)
+        _, f0 = s.residual()
+        s.tile = 1
+        assert s.tile == 1 and s.stage_waves in (2, 3)
+        for w in (2, 3):
+            s.stage_waves = w
+            R_g, f_g = s.residual()
+            ef, eR = rel(f_g, f_o), rel(R_g, R_o)
+            print("  waves %d: rel err f %.3e  R %.3e, vs one-row kernel %.3e"
+                  % (w, ef, eR, rel(f_g, f0)), flush=True)
+            assert ef < RES_TOL and eR < RES_TOL
+            assert rel(f_g, f0) < 1e-13
+        n_g = s.step(case.dt, case.scheme, case.nsteps)
+        U_g = s.get_state()
+    eU = rel(U_g, U_o)
+    en = float(np.max(np.abs(n_g - n_o) / np.abs(n_o)))
+    print("  rel err U %.3e  norms %.3e" % (eU, en), flush=True)
+    assert eU < SOL_TOL and en < SOL_TOL
+
+
+def test_tile_selection_rules(built):
+    """Tile 1 exists for q = 1 only and for 2 and 3 waves per SIMD: selecting
+    it moves a target of 1 to 2; q = 2 reports DGX_ERR_UNSUPPORTED."""
+    c = CASES["channel_p2"]()
+    with dgx.Solver(c.mesh, c.params) as s:
+        s.stage_waves = 1
+        s.tile = 1
+        assert s.stage_waves == 2
+        with pytest.raises(dgx.DgxError):
+            s.stage_waves = 1
+        s.tile = 0
+        s.stage_waves = 1
+        tile, waves, us = s.autotune_tiles(reps=1)
+        assert tile in (0, 1) and waves in (1, 2, 3)
+        assert us[1][0] == 0.0 and np.all(us[0] > 0) and np.all(us[1][1:] > 0)
+    c = CASES["channel_q2_p2"]()
+    with dgx.Solver(c.mesh, c.params) as s:
+        with pytest.raises(dgx.DgxError) as ei:
+            s.tile = 1
+        assert ei.value.status == dgx.ERR_UNSUPPORTED
