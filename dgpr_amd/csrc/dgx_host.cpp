@@ -77,50 +77,46 @@ dgx_status dgx_bump_nodes(uint64_t q, uint64_t nx, uint64_t ny, uint64_t nz, dou
 {
   if (!nodes || q < 1 || nx == 0 || ny == 0 || nz == 0)
     return DGX_ERR_INVALID;
-  uint64_t qp1 = q + 1;
-  double elx = lx / (double)nx, ely = ly / (double)ny, elz = lz / (double)nz;
-  double nsx = elx / (double)q, nsz = elz / (double)q;
-  double gly = ly;
-  for (uint64_t ez = 0; ez < nz; ++ez)
+  // The arithmetic below is the reference's node formula, expression for
+  // expression, so the nodes (and every result built on them) agree bit for
+  // bit with generate_bump_nodes (geometry_common.cpp:494-576):
+  //  * element edges are uniform in x and z; in y they are uniform (a = 0)
+  //    or tanh-clustered towards both walls, edge k of ny at
+  //    ly ((tanh((2k/ny - 1) atanh a) / 2) (2/a) + 1) / 2;
+  //  * the q + 1 nodes per direction split each element uniformly;
+  //  * the straight y is then blended linearly from the Gaussian bump
+  //    b(x) = h exp(-(x - c)^2 / (2 v)) at y = 0 to the flat wall at y = ly.
+  const double hx = lx / (double)nx, hy = ly / (double)ny, hz = lz / (double)nz;
+  std::vector<double> yedge(ny + 1);
+  for (uint64_t k = 0; k <= ny; ++k)
   {
-    double esz = ez * elz;
-    for (uint64_t ey = 0; ey < ny; ++ey)
+    if (a != 0.0)
     {
-      double esy, eey;
-      if (a != 0.0)
-      {
-        double xi = -1. + 2. * ((double)ey / ny);
-        double y  = 0.5 * tanh(xi * atanh(a));
-        esy = gly * ((y * (2. / a) + 1.) / 2.);
-        xi  = -1. + 2. * (((double)(ey + 1.)) / ny);
-        y   = 0.5 * tanh(xi * atanh(a));
-        eey = gly * ((y * (2. / a) + 1.) / 2.);
-      }
-      else
-      {
-        esy = ey * ely;
-        eey = (ey + 1.) * ely;
-      }
-      double nsy = (eey - esy) / (double)q;
-      for (uint64_t ex = 0; ex < nx; ++ex)
-      {
-        double esx = ex * elx;
-        uint64_t e = nx * ny * ez + nx * ey + ex;
-        for (uint64_t k = 0; k < qp1; ++k)
-          for (uint64_t j = 0; j < qp1; ++j)
-            for (uint64_t i = 0; i < qp1; ++i)
-            {
-              double* nd = nodes + (e * qp1 * qp1 * qp1 + qp1 * qp1 * k + qp1 * j + i) * 3;
-              double x = esx + nsx * (double)i;
-              double ylin = esy + nsy * (double)j;
-              // bump on the lower wall, linearly blended to zero at y = ly
-              double func = bmp_height * exp(-((x - bmp_center) * (x - bmp_center)) /
-                                             (2. * bmp_variance));
-              nd[0] = x;
-              nd[1] = ((ylin - 0.) / (gly - 0.)) * (gly - func) + func;
-              nd[2] = esz + nsz * (double)k;
-            }
-      }
+      const double t = -1. + 2. * ((double)k / ny);
+      const double c = 0.5 * tanh(t * atanh(a));
+      yedge[k] = ly * ((c * (2. / a) + 1.) / 2.);
+    }
+    else
+      yedge[k] = k * hy;
+  }
+  const uint64_t m = q + 1, npe = m * m * m;
+  const double dx = hx / (double)q, dz = hz / (double)q;
+  for (uint64_t e = 0; e < nx * ny * nz; ++e)
+  {
+    const uint64_t ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    const double x0 = ex * hx, y0 = yedge[ey], z0 = ez * hz;
+    const double dy = (yedge[ey + 1] - y0) / (double)q;
+    double* X = nodes + e * npe * 3;
+    for (uint64_t n = 0; n < npe; ++n)
+    {
+      const uint64_t i = n % m, j = (n / m) % m, k = n / (m * m);
+      const double x = x0 + dx * (double)i;
+      const double ystraight = y0 + dy * (double)j;
+      const double bump =
+          bmp_height * exp(-((x - bmp_center) * (x - bmp_center)) / (2. * bmp_variance));
+      X[3 * n + 0] = x;
+      X[3 * n + 1] = ((ystraight - 0.) / (ly - 0.)) * (ly - bump) + bump;
+      X[3 * n + 2] = z0 + dz * (double)k;
     }
   }
   return DGX_OK;

@@ -12,9 +12,12 @@
    container only), ``oracle/ref_harness.cpp`` compiled against them, with
    the image's MPICH, into ``oracle/_ref/ref_harness``: the reference's own
    residual and RK steps, used to write the golden vectors under
-   ``tests/golden/`` (``tests/golden/make_reference_residuals.py``).  The
-   reference's build system is not used: one g++ line on its unity sources,
-   as its Makefile's ``dg`` target compiles solver/dg.cpp.
+   ``tests/golden/`` (``tests/golden/make_reference_residuals.py``); and the
+   reference's CPU driver itself (solver/dg.cpp) into ``oracle/_ref/dg``,
+   whose output files give the driver-level golden vectors
+   (``tests/golden/make_reference_driver_outputs.py``).  The reference's
+   build system is not used: one g++ line each on its unity sources, as its
+   Makefile's ``dg`` target compiles solver/dg.cpp.
 """
 import os
 import subprocess
@@ -29,6 +32,7 @@ REF_SOLVER = "/root/reference/solver"
 MPI_PREFIX = "/opt/conda"
 REF_OUT_DIR = os.path.join(HERE, "_ref")
 REF_HARNESS = os.path.join(REF_OUT_DIR, "ref_harness")
+REF_DG = os.path.join(REF_OUT_DIR, "dg")
 
 
 def _cc(out, defines, verbose):
@@ -46,19 +50,25 @@ def reference_available() -> bool:
             and os.path.isfile(os.path.join(MPI_PREFIX, "include", "mpi.h")))
 
 
-def build_reference(verbose: bool = False) -> str:
-    """Compile the reference harness into oracle/_ref (only where the
-    reference sources are present); returns its path."""
-    os.makedirs(REF_OUT_DIR, exist_ok=True)
+def _ref_cxx(src, out, verbose):
     lib = os.path.join(MPI_PREFIX, "lib")
     cmd = ["g++", "-std=c++11", "-O2", "-I" + REF_SOLVER,
            "-I" + os.path.join(MPI_PREFIX, "include"),
-           "-o", REF_HARNESS + ".tmp", os.path.join(HERE, "ref_harness.cpp"),
+           "-o", out + ".tmp", src,
            "-L" + lib, "-Wl,-rpath," + lib, "-lmpicxx", "-lmpi"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(REF_HARNESS + ".tmp", REF_HARNESS)
+    os.replace(out + ".tmp", out)
+
+
+def build_reference(verbose: bool = False) -> str:
+    """Compile the reference harness and the reference's CPU driver into
+    oracle/_ref (only where the reference sources are present); returns the
+    harness path."""
+    os.makedirs(REF_OUT_DIR, exist_ok=True)
+    _ref_cxx(os.path.join(HERE, "ref_harness.cpp"), REF_HARNESS, verbose)
+    _ref_cxx(os.path.join(REF_SOLVER, "dg.cpp"), REF_DG, verbose)
     return REF_HARNESS
 
 

@@ -187,3 +187,27 @@ def test_driver_running_mean_and_restart(orc, driver, tmp_path):
     ts, _, outs2 = dgx.read_dg_outputs(out2 + ".dg")
     assert ts == 5 and list(outs2) == ["state", "mean"]
     assert rel(outs2["state"], states[5]) < TOL and rel(outs2["mean"], m) < TOL
+
+
+def test_driver_matches_reference_driver_with_mean(driver, tmp_path):
+    """The options of the reference's own CPU driver run held in
+    tests/golden/reference_driver_outputs.* (solver/dg.cpp: turbulent start on
+    a stretched bump channel, three RK4 steps, running mean from step 1):
+    dgx_drive prints the same norms and writes "state" and "mean" in the same
+    order and to the same values (1e-10 relative)."""
+    import json
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    meta = json.load(open(os.path.join(gdir, "reference_driver_outputs.json")))["cases"]
+    z = np.load(os.path.join(gdir, "reference_driver_outputs.npz"), allow_pickle=False)
+    out = str(tmp_path / "ref_opts")
+    r, norms = run(driver, meta["mean_run"]["args"] + ["-ofile", out])
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref_norms = z["mean_run/norms"]
+    assert sorted(norms) == list(range(len(ref_norms)))
+    en = max(abs(norms[t] - ref_norms[t]) / ref_norms[t] for t in range(len(ref_norms)))
+    ts, mesh, outs = dgx.read_dg_outputs(out + ".dg")
+    eU, eM = rel(outs["state"], z["mean_run/state"]), rel(outs["mean"], z["mean_run/mean"])
+    print("  norms %.2e  state %.2e  mean %.2e" % (en, eU, eM), flush=True)
+    assert ts == meta["mean_run"]["tstep"] and list(outs) == meta["mean_run"]["keys"]
+    assert np.array_equal(mesh.nodes, z["mean_run/nodes"])
+    assert en < TOL and eU < TOL and eM < TOL
