@@ -119,35 +119,44 @@ def cpu_baseline(seconds_hint=20.0, threads=None):
 
 def run_ensemble(a, h):
     """64 independent periodic cubes (sizes, p, Mach, Re from seed 3) advanced
-    together: per RK stage one batched launch per (p, q) group, one RK step
-    of the whole ensemble replayed from a captured hipGraph, occupancy per
-    group from the autotuner.  Compared with the same cases stepped one after
-    the other with the single-case stepper ("64 sequential plain launches").
-    The step count is the workload's (50); --steps overrides it."""
+    together.  Default mode: per RK stage ONE launch of the persistent stage
+    kernel for all 64 members (p = 1..3 in one kernel, workgroups taking
+    work units from an atomic counter), one RK step of the whole ensemble
+    replayed from a captured hipGraph, occupancy target and tile (elements
+    per workgroup, nodes per lane) from the autotuner.  Also timed: the
+    batched mode (one launch per (p, q) group and stage, autotuned), and the
+    same cases stepped one after the other with the single-case stepper
+    ("64 sequential plain launches").  The step count is the workload's
+    (50); --steps overrides it."""
     cs = cases.config3()
     steps = a.steps if a.steps_given else cs[0].nsteps
     dts = [c.dt for c in cs]
     dof = sum(c.dof for c in cs)
+    stream = None
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    h.hipEventCreate(C.byref(ev0))
+    h.hipEventCreate(C.byref(ev1))
+    res = {}
     with dgx.Ensemble([c.mesh for c in cs], [c.params for c in cs]) as en:
-        for m, c in zip(en.members, cs):
-            m.set_state(c.U0)
-        waves = en.autotune("RK4", dts, reps=2) if a.autotune else None
-        en.step("RK4", dts, max(a.warmup, 1), graph=True)
-        for m, c in zip(en.members, cs):
-            m.set_state(c.U0)
-        en.step("RK4", dts, 0, graph=True)  # traces of the reset states, untimed
         stream = C.c_void_p(en.stream())
-        ev0, ev1 = C.c_void_p(), C.c_void_p()
-        h.hipEventCreate(C.byref(ev0))
-        h.hipEventCreate(C.byref(ev1))
-        h.hipEventRecord(ev0, stream)
-        en.step("RK4", dts, steps, graph=True, sync=False)
-        h.hipEventRecord(ev1, stream)
-        h.hipEventSynchronize(ev1)
-        en.sync()
-        ms = C.c_float()
-        h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
-        t_ens = ms.value * 1e-3
+        for mode in (False, True):  # the default (persistent) last, its states are checked
+            en.set_persistent(mode)
+            for m, c in zip(en.members, cs):
+                m.set_state(c.U0)
+            waves = en.autotune("RK4", dts, reps=2) if a.autotune else None
+            tuning = sorted(set(en.member_tuning(i) for i in range(len(cs))))
+            en.step("RK4", dts, max(a.warmup, 1), graph=True)
+            for m, c in zip(en.members, cs):
+                m.set_state(c.U0)
+            en.step("RK4", dts, 0, graph=True)  # traces of the reset states, untimed
+            h.hipEventRecord(ev0, stream)
+            en.step("RK4", dts, steps, graph=True, sync=False)
+            h.hipEventRecord(ev1, stream)
+            h.hipEventSynchronize(ev1)
+            en.sync()
+            ms = C.c_float()
+            h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+            res[mode] = (ms.value * 1e-3, waves, tuning)
         finite = all(bool(dgx.np.isfinite(m.get_state()).all()) for m in en.members)
         # the same cases one after the other, single-case launches
         for m, c in zip(en.members, cs):
@@ -161,6 +170,8 @@ def run_ensemble(a, h):
             m.sync()
         h.hipDeviceSynchronize()
         t_seq = time.perf_counter() - t0
+    t_ens, waves, tuning = res[True]
+    t_bat = res[False][0]
     work = dof * 4 * steps
     value = work / t_ens / 1e9
     bpd = 170.0  # RK4 with accumulator, affine meshes (no geometry bytes)
@@ -185,7 +196,14 @@ def run_ensemble(a, h):
             "cases": len(cs), "dof": dof, "scheme": "RK4",
             "groups": sorted(set((c.mesh.p, c.mesh.q) for c in cs)),
             "stage_waves_per_member": waves,
-            "parallelism": "single GPU, batched launches + hipGraph",
+            "launch_tuning": [{"persistent": p_, "tile": t_, "waves": w_} for p_, t_, w_ in tuning],
+            "parallelism": "single GPU, one persistent launch per stage + hipGraph",
+        },
+        "batched_per_group": {
+            "value": dof * 4 * steps / t_bat / 1e9,
+            "unit": "GDOF-stage/s",
+            "ms_per_step": 1e3 * t_bat / steps,
+            "what": "one batched launch per (p, q) group and stage, autotuned, hipGraph",
         },
         "sequential": {
             "value": work / t_seq / 1e9,
@@ -196,7 +214,7 @@ def run_ensemble(a, h):
         "speedup_vs_sequential": t_seq / t_ens,
         "roofline": {
             "bound": "hbm",
-            "kernel": "dgx_rhs_batch_kernel<P,Q,W> (all groups)",
+            "kernel": "dgx_ens_stage_kernel<W,TILE,AFF> (all members, one launch per stage)",
             "achieved": bpd * work / t_ens / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -352,7 +370,7 @@ def timed_steps(h, s, case, nsteps, stream, ev0, ev1, norms_dev=None):
     h.hipDeviceSynchronize()
     h.hipEventRecord(ev0, stream)
     if norms_dev is not None:
-        s.step_norms_async(case.dt, case.scheme, nsteps, norms_dev)
+        s.step_norms_async(case.dt, case.scheme, nsteps, norms_dev.value)
     else:
         s.step_async(case.dt, case.scheme, nsteps)
     h.hipEventRecord(ev1, stream)

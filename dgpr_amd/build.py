@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "libdgx.so")
 DRIVER = os.path.join(HERE, "dgx_drive")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["dgx_kernels.hip", "dgx_api.cpp", "dgx_ensemble.cpp", "dgx_adjoint.cpp",
+SOURCES = ["dgx_kernels.hip", "dgx_ensemble_kernels.hip", "dgx_api.cpp", "dgx_ensemble.cpp", "dgx_adjoint.cpp",
            "dgx_tables.cpp", "dgx_host.cpp"]
 ARCH = os.environ.get("DGX_ARCH", "gfx950")
 
@@ -33,7 +33,7 @@ def _run(cmd, verbose):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose: bool = False, jobs: int = 4, out: str = OUT, defines=()) -> str:
+def build(verbose: bool = False, jobs: int = 7, out: str = OUT, defines=()) -> str:
     """Compile every source and link ``out``; ``defines`` are extra -D flags
     (for experiments; the shipped library needs none)."""
     build_dir = BUILD if out == OUT else out + ".objs"

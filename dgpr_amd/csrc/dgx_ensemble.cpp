@@ -2,15 +2,24 @@
 //
 // The reference runs one case per process (dg.cpp / cudg.cu).  An ensemble
 // keeps one ordinary context per member (mesh, state, stage buffers, trace
-// records, lifting coefficients) and advances all members together: for every
-// RK stage, the members that share (p, q) run as ONE launch of the batched
-// stage kernel (each grid block finds its member in a sorted item table), so
-// small members no longer leave most of the GPU idle.  One RK step of the
-// whole ensemble (stages x groups launches) can be captured in a hipGraph and
-// replayed.  Each group's occupancy target is chosen by timing
-// (dgx_ensemble_autotune).
+// records, lifting coefficients) and advances all members together.  Per RK
+// stage:
+//  * persistent mode (default): all q = 1 members, of every order, run as
+//    ONE launch of the persistent stage kernel (dgx_ensemble_kernels.hip),
+//    whose workgroups take work units (rows or tiles of one member's
+//    elements) from an atomic counter, largest members first; affine and
+//    curved members form two such launches, q > 1 members one batched launch
+//    per (p, q);
+//  * batched mode: one launch of the batched stage kernel per (p, q, affine)
+//    group (each grid block finds its member in a sorted item table).
+// One RK step of the whole ensemble can be captured in a hipGraph and
+// replayed (kernels only: the unit counters are cleared by a kernel).  The
+// autotuner picks, per launch group, the occupancy target and, for the
+// persistent kernel, the tile (one-row units or tiled units: elements per
+// workgroup and nodes per lane), by timing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -26,11 +35,14 @@ namespace
 {
 struct EnsGroup
 {
-  int p, q;
-  int affine;  // every member an affine q = 1 mesh
+  int p, q;      // p = 0 for a persistent group (q = 1 members of any order)
+  int affine;    // every member an affine q = 1 mesh
+  bool persistent;
   std::vector<int> members;
-  int nblocks;
+  int nblocks;   // grid blocks (batched) or work units (persistent)
   int waves;
+  int tile;      // persistent groups: 0 one-row units, 1 tiled units
+  int grid;      // persistent groups: resident workgroups launched
   BatchItem* items[4];  // device item tables, one per stage
 };
 }  // namespace
@@ -48,6 +60,9 @@ struct dgx_ensemble
   int nstages;
   std::vector<double> dt;
   std::vector<Phys> phys;  // each member's physics when the items were built
+  bool persistent;         // run q = 1 members through the persistent kernel
+  int* counters;           // persistent-kernel unit counters [group][stage]
+  int ncu;                 // compute units of the device
   bool graph_ready;
   hipGraph_t graph;
   hipGraphExec_t exec;
@@ -82,6 +97,12 @@ static void drop_graph(dgx_ensemble* en)
     (void)hipGraphDestroy(en->graph);
     en->graph_ready = false;
   }
+}
+
+// Whether group g runs in the current mode.
+static bool active(const dgx_ensemble* en, const EnsGroup& g)
+{
+  return en->persistent ? (g.persistent || g.q != 1) : !g.persistent;
 }
 
 // True when some member's physics changed (dgx_set_params on a member
@@ -122,11 +143,17 @@ static dgx_status plan(dgx_ensemble* en, int scheme, const double* dt)
         it.sa = sa[s];
         it.lam = c->lam;
         it.block0 = block0;
+        it.p = c->p;
         host[s].push_back(it);
       }
-      block0 += stage_blocks(c->p, c->nelem);
+      block0 += g.persistent ? ens_units(c->p, c->nelem, g.tile) : stage_blocks(c->p, c->nelem);
     }
     g.nblocks = block0;
+    if (g.persistent)
+    {
+      const int per_cu = ens_blocks_per_cu(g.waves, g.tile, g.affine != 0);
+      g.grid = std::max(1, std::min(block0, en->ncu * std::max(per_cu, 1)));
+    }
     EHIP(en, hipStreamSynchronize(en->stream));  // no launch may still read the tables
     for (int s = 0; s < ns; ++s)
       EHIP(en, hipMemcpy(g.items[s], host[s].data(), sizeof(BatchItem) * host[s].size(),
@@ -149,16 +176,34 @@ static hipError_t enqueue_stage(dgx_ensemble* en, int s, int only_group = -1)
     if (only_group >= 0 && gi != only_group)
       continue;
     EnsGroup& g = en->groups[gi];
-    hipError_t e = rhs_batch_dispatch(g.p, g.q, g.items[s], (int)g.members.size(), g.nblocks,
-                                      en->status, g.waves, g.affine != 0, en->stream);
+    if (!active(en, g))
+      continue;
+    hipError_t e = g.persistent
+                       ? ens_stage_dispatch(g.items[s], (int)g.members.size(), g.nblocks,
+                                            en->counters + 4 * gi + s, en->status, g.waves,
+                                            g.tile, g.affine != 0, g.grid, en->stream)
+                       : rhs_batch_dispatch(g.p, g.q, g.items[s], (int)g.members.size(),
+                                            g.nblocks, en->status, g.waves, g.affine != 0,
+                                            en->stream);
     if (e != hipSuccess)
       return e;
   }
   return hipSuccess;
 }
 
+static hipError_t reset_counters(dgx_ensemble* en)
+{
+  return ens_reset_dispatch(en->counters, 4 * (int)en->groups.size(), en->stream);
+}
+
 static hipError_t enqueue_step(dgx_ensemble* en)
 {
+  if (en->persistent)
+  {
+    hipError_t e = reset_counters(en);
+    if (e != hipSuccess)
+      return e;
+  }
   for (int s = 0; s < en->nstages; ++s)
   {
     hipError_t e = enqueue_stage(en, s);
@@ -192,6 +237,7 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
   en->planned = false;
   en->graph_ready = false;
   en->status = nullptr;
+  en->counters = nullptr;
   en->stream = nullptr;
   en->err[0] = 0;
   for (int i = 0; i < ncase; ++i)
@@ -207,27 +253,52 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
     en->ctx.push_back(c);
   }
   en->device = en->ctx[0]->device;
+  en->persistent = true;
+  en->counters = nullptr;
+  en->ncu = 1;
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, en->device) ==
+            hipSuccess &&
+        ncu > 0)
+      en->ncu = ncu;
+  }
+  auto group_of = [&](bool persistent, int p, int q, int affine) -> EnsGroup& {
+    for (EnsGroup& x : en->groups)
+      if (x.persistent == persistent && x.p == p && x.q == q && x.affine == affine)
+        return x;
+    EnsGroup ng;
+    ng.p = p;
+    ng.q = q;
+    ng.affine = affine;
+    ng.persistent = persistent;
+    ng.nblocks = 0;
+    ng.waves = persistent ? 3 : default_stage_waves(p, q);
+    ng.tile = 0;
+    ng.grid = 1;
+    for (int s = 0; s < 4; ++s)
+      ng.items[s] = nullptr;
+    en->groups.push_back(ng);
+    return en->groups.back();
+  };
+  // batched groups per (p, q, affine); persistent groups per affine class of
+  // the q = 1 members, largest members (most elements x nodes) first
   for (int i = 0; i < ncase; ++i)
   {
     dgx_ctx* c = en->ctx[i];
-    EnsGroup* g = nullptr;
-    for (EnsGroup& x : en->groups)
-      if (x.p == c->p && x.q == c->q && x.affine == c->affine)
-        g = &x;
-    if (!g)
-    {
-      EnsGroup ng;
-      ng.p = c->p;
-      ng.q = c->q;
-      ng.affine = c->affine;
-      ng.nblocks = 0;
-      ng.waves = default_stage_waves(c->p, c->q);
-      for (int s = 0; s < 4; ++s)
-        ng.items[s] = nullptr;
-      en->groups.push_back(ng);
-      g = &en->groups.back();
-    }
-    g->members.push_back(i);
+    group_of(false, c->p, c->q, c->affine).members.push_back(i);
+  }
+  std::vector<int> order(ncase);
+  for (int i = 0; i < ncase; ++i)
+    order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (size_t)en->ctx[a]->nelem * en->ctx[a]->nbf > (size_t)en->ctx[b]->nelem * en->ctx[b]->nbf;
+  });
+  for (int i : order)
+  {
+    dgx_ctx* c = en->ctx[i];
+    if (c->q == 1)
+      group_of(true, 0, 1, c->affine).members.push_back(i);
   }
   auto bail = [&](const char* what, hipError_t e) {
     snprintf(g_ens_err, sizeof(g_ens_err), "%s failed: %s", what, hipGetErrorString(e));
@@ -243,6 +314,8 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
         return bail("hipMalloc(items)", e);
   if ((e = hipMalloc(&en->status, sizeof(int))) != hipSuccess)
     return bail("hipMalloc(status)", e);
+  if ((e = hipMalloc(&en->counters, sizeof(int) * 4 * en->groups.size())) != hipSuccess)
+    return bail("hipMalloc(counters)", e);
   if ((e = hipMemsetAsync(en->status, 0, sizeof(int), en->stream)) != hipSuccess)
     return bail("hipMemset(status)", e);
   if ((e = hipStreamSynchronize(en->stream)) != hipSuccess)
@@ -264,6 +337,8 @@ dgx_status dgx_ensemble_destroy(dgx_ensemble* en)
         (void)hipFree(g.items[s]);
   if (en->status)
     (void)hipFree(en->status);
+  if (en->counters)
+    (void)hipFree(en->counters);
   for (dgx_ctx* c : en->ctx)
     dgx_destroy(c);
   if (en->stream)
@@ -371,38 +446,98 @@ dgx_status dgx_ensemble_autotune(dgx_ensemble* en, int scheme, const double* dt,
   hipEvent_t e0, e1;
   EHIP(en, hipEventCreate(&e0));
   EHIP(en, hipEventCreate(&e1));
-  // stage 1 reads U and its records and writes only stage buffers
+  // stage 1 reads U and its records and writes only stage buffers; a
+  // persistent launch needs its unit counter cleared first
+  auto one = [&](int gi) -> hipError_t {
+    hipError_t e = en->groups[gi].persistent ? reset_counters(en) : hipSuccess;
+    return e == hipSuccess ? enqueue_stage(en, 0, gi) : e;
+  };
   for (int gi = 0; gi < (int)en->groups.size(); ++gi)
   {
     EnsGroup& g = en->groups[gi];
-    int best = g.waves;
+    if (!active(en, g))
+      continue;
+    int best_w = g.waves, best_t = g.tile;
     double best_ms = 0.0;
-    for (int w = 1; w <= 3; ++w)
-    {
-      g.waves = w;
-      EHIP(en, enqueue_stage(en, 0, gi));
-      EHIP(en, hipEventRecord(e0, en->stream));
-      for (int r = 0; r < reps; ++r)
-        EHIP(en, enqueue_stage(en, 0, gi));
-      EHIP(en, hipEventRecord(e1, en->stream));
-      EHIP(en, hipEventSynchronize(e1));
-      float ms = 0.0f;
-      EHIP(en, hipEventElapsedTime(&ms, e0, e1));
-      if (w == 1 || ms < best_ms)
+    bool have = false;
+    for (int t = 0; t <= (g.persistent ? 1 : 0); ++t)
+      for (int w = g.persistent ? 2 : 1; w <= 3; ++w)
       {
-        best_ms = ms;
-        best = w;
+        g.waves = w;
+        if (g.persistent)
+        {
+          g.tile = t;
+          en->planned = false;  // units and grid depend on the tile and target
+          dgx_status ps = plan(en, scheme, dt);
+          if (ps != DGX_OK)
+            return ps;
+        }
+        EHIP(en, one(gi));
+        EHIP(en, hipEventRecord(e0, en->stream));
+        for (int r = 0; r < reps; ++r)
+          EHIP(en, one(gi));
+        EHIP(en, hipEventRecord(e1, en->stream));
+        EHIP(en, hipEventSynchronize(e1));
+        float ms = 0.0f;
+        EHIP(en, hipEventElapsedTime(&ms, e0, e1));
+        if (!have || ms < best_ms)
+        {
+          best_ms = ms;
+          best_w = w;
+          best_t = t;
+          have = true;
+        }
       }
+    g.waves = best_w;
+    g.tile  = best_t;
+    if (g.persistent)
+    {
+      en->planned = false;
+      dgx_status ps = plan(en, scheme, dt);
+      if (ps != DGX_OK)
+        return ps;
     }
-    g.waves = best;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (waves_out)
     for (const EnsGroup& g : en->groups)
-      for (int m : g.members)
-        waves_out[m] = g.waves;
+      if (active(en, g))
+        for (int m : g.members)
+          waves_out[m] = g.waves;
   return dgx_ensemble_sync(en);
+}
+
+dgx_status dgx_ensemble_set_persistent(dgx_ensemble* en, int on)
+{
+  if (!en)
+    return DGX_ERR_INVALID;
+  if ((on != 0) != en->persistent)
+  {
+    EHIP(en, hipStreamSynchronize(en->stream));
+    drop_graph(en);
+    en->persistent = on != 0;
+  }
+  return DGX_OK;
+}
+
+dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* en, int i, int* persistent, int* tile,
+                                      int* waves)
+{
+  if (!en || i < 0 || i >= (int)en->ctx.size())
+    return DGX_ERR_INVALID;
+  for (const EnsGroup& g : en->groups)
+    if (active(en, g) && std::find(g.members.begin(), g.members.end(), i) != g.members.end())
+    {
+      if (persistent)
+        *persistent = g.persistent ? 1 : 0;
+      if (tile)
+        *tile = g.persistent ? g.tile : 0;
+      if (waves)
+        *waves = g.waves;
+      return DGX_OK;
+    }
+  return DGX_ERR_INVALID;
 }
 
 const char* dgx_ensemble_last_error(const dgx_ensemble* en)

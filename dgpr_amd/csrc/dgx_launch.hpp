@@ -78,7 +78,9 @@ struct StageArgs
 };
 
 // One member of a batched stage launch: its mesh, physics, stage arguments
-// and lifting coefficients, and the first grid block of its elements.
+// and lifting coefficients, the first grid block (or work unit) of its
+// elements, and its order (read by the persistent kernel, which holds the
+// bodies of every order).
 struct BatchItem
 {
   MeshArgs mesh;
@@ -86,6 +88,7 @@ struct BatchItem
   StageArgs sa;
   const double* lam;
   int block0;
+  int p;
 };
 
 // waves: occupancy target of the compiled stage kernel (1, 2 or 3 waves per
@@ -103,6 +106,23 @@ hipError_t rhs_batch_dispatch(int p, int q, const BatchItem* items_dev, int nite
                               int* status, int waves, bool affine, hipStream_t st);
 // grid blocks of one mesh in a stage launch
 int stage_blocks(int p, int nelem);
+
+// ---- persistent ensemble stage kernel (dgx_ensemble_kernels.hip) ----------
+// One launch works through the units (tiles of elements) of every member of
+// a q = 1 group, of any order p = 1..3: each 64-lane workgroup takes the next
+// unit from *counter (atomic) until all nunits are done.  items_dev sorted
+// by first unit (block0), which ens_units counts per member.  tile 0: units
+// of one row (the one-row body); tile 1: units of the tiled body.  affine:
+// every member affine.  counter must read 0 at launch (ens_reset_dispatch).
+int ens_units(int p, int nelem, int tile);
+hipError_t ens_stage_dispatch(const BatchItem* items_dev, int nitems, int nunits, int* counter,
+                              int* status, int waves, int tile, bool affine, int grid,
+                              hipStream_t st);
+// resident workgroups per CU of the persistent kernel (occupancy query)
+int ens_blocks_per_cu(int waves, int tile, bool affine);
+// counters[0..n) = 0 with a kernel (no memset node in captured graphs)
+hipError_t ens_reset_dispatch(int* counters, int n, hipStream_t st);
+hipError_t upload_tables_ensemble(const PTab* pt, const QTab* qt);
 // face trace records Tout of the Gauss-nodal state V (and the boundary
 // states of its domain-boundary faces, see trace_doubles)
 hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,

@@ -18,9 +18,19 @@
 // LDS per face point: the tangential dual-consistency fluxes (8 doubles),
 // then the projected face term and the normal flux (9 doubles), in one
 // 64 x 9 region shared with the volume divergence and the trace exchange.
-// Included by dgx_kernels.hip inside namespace dgx, after stage_body and its
+// Included by dgx_stage.hpp inside namespace dgx, after stage_body and its
 // helpers.
 #pragma once
+
+// rows per tile of the tiled stage kernel at each order (0: none compiled):
+// p = 3: two one-element rows, so each face subpass is the 64 points of one
+// direction of two elements; p = 1: one row of 8 elements (16 faces of 4
+// points per direction); p = 2: three rows of 2 elements (12 faces of 9
+// points per direction in two subpasses)
+constexpr int tile_rows(int p)
+{
+  return p == 3 ? 2 : (p == 2 ? 3 : (p == 1 ? 1 : 0));
+}
 
 template <int P, int R>
 struct TileDims
@@ -103,10 +113,8 @@ __device__ __forceinline__ void tile_face_subpass(
     gi[D]  = 0;
     gi[T1] = a;
     gi[T2] = b;
-    double J[3][3], JiL[3][3], detL;
-    geometry_at<Q, AFF>(qt, sX[l], sG[l], D, hi, gi[0], gi[1], gi[2], J, JiL, detL);
-    double nv[3];
-    const double fJ   = face_normal(J, D, hi, nv);
+    double JiL[3][3], nv[3];
+    const double fJ = face_geometry<Q, AFF>(qt, sX[l], sG[l], D, hi, gi[0], gi[1], gi[2], JiL, nv);
     const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
     double Ub[5], vgR[3] = {0.0, 0.0, 0.0};
     double lamR = 0.0;
@@ -271,10 +279,8 @@ __device__ __forceinline__ void tile_trace_subpass(
     gi[D]  = 0;
     gi[T1] = a;
     gi[T2] = b;
-    double J[3][3], Ji[3][3], det;
-    geometry_at<Q, AFF>(qt, sX[l], sG[l], D, hi, gi[0], gi[1], gi[2], J, Ji, det);
-    double nv[3];
-    face_normal(J, D, hi, nv);
+    double Ji[3][3], nv[3];
+    face_geometry<Q, AFF>(qt, sX[l], sG[l], D, hi, gi[0], gi[1], gi[2], Ji, nv);
     double gr[3][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
@@ -339,31 +345,53 @@ __device__ __forceinline__ void tile_trace_direction(
                                             Tout, ok);
 }
 
-// The tiled stage: f_g of the tile's E elements fused with the RK stage of
-// StageArgs, then the trace records of the new state (see stage_body for
-// the meaning of every argument; the arithmetic per point is the same).
-template <int P, int Q, int R, int W, typename TT, bool AFF>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
-void dgx_tile_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
-                           int* status)
+// LDS of one tile, carved from a byte buffer (see StageLds).
+template <int P, int Q, int R>
+struct TileLds
 {
   using TD = TileDims<P, R>;
+  static constexpr int E = TD::E, NB = TD::NB, NG = (Q + 1) * (Q + 1) * (Q + 1);
+  static constexpr size_t oV  = 0;
+  static constexpr size_t oX  = oV + sizeof(double) * E * 5 * NB;
+  static constexpr size_t oG  = oX + sizeof(double) * E * NG * 3;
+  static constexpr size_t oR  = oG + sizeof(double) * E * GREC;
+  static constexpr size_t oPT = oR + sizeof(double) * TD::SR;
+  static constexpr size_t oQT = oPT + sizeof(PTab);
+  static constexpr size_t oNb = oQT + sizeof(QTab);
+  static constexpr size_t oBt = oNb + sizeof(int) * E * 6;
+  static constexpr size_t bytes = oBt + sizeof(int) * E * 6;
+};
+
+// The tiled stage: f_g of the tile's E elements (tile number `block`)
+// fused with the RK stage of StageArgs, then the trace records of the new
+// state (see stage_body_lds for the meaning of every argument; the
+// arithmetic per point is the same), in the LDS region lds
+// (TileLds<P, Q, R>::bytes, 16-byte aligned).
+template <int P, int Q, int R, typename TT, bool AFF>
+__device__ __forceinline__ void tile_body_lds(const MeshArgs& mesh, const Phys& ph,
+                                              const StageArgs& sa,
+                                              const double* __restrict__ lam, int* status,
+                                              int block, unsigned char* lds)
+{
+  using TD = TileDims<P, R>;
+  using TL = TileLds<P, Q, R>;
   constexpr int N = TD::N, NB = TD::NB, EPR = TD::EPR, E = TD::E;
   constexpr int NG = (Q + 1) * (Q + 1) * (Q + 1);
 
-  __shared__ double sV[E][5][NB];
-  __shared__ double sX[E][NG * 3];
-  __shared__ double sG[E][GREC];
-  __shared__ double sR[TD::SR];
-  __shared__ PTab sPT;
-  __shared__ QTab sQT;
-  __shared__ int sNb[E][6], sBt[E][6];
+  auto sV  = reinterpret_cast<double(*)[5][NB]>(lds + TL::oV);
+  auto sX  = reinterpret_cast<double(*)[NG * 3]>(lds + TL::oX);
+  auto sG  = reinterpret_cast<double(*)[GREC]>(lds + TL::oG);
+  double* sR = reinterpret_cast<double*>(lds + TL::oR);
+  PTab& sPT = *reinterpret_cast<PTab*>(lds + TL::oPT);
+  QTab& sQT = *reinterpret_cast<QTab*>(lds + TL::oQT);
+  auto sNb = reinterpret_cast<int(*)[6]>(lds + TL::oNb);
+  auto sBt = reinterpret_cast<int(*)[6]>(lds + TL::oBt);
   stage_tables<P, Q>(&sPT, &sQT);
   const PTab& pt = sPT;
   const QTab& qt = sQT;
 
   const int tid = threadIdx.x;
-  const int e0  = blockIdx.x * E;
+  const int e0  = block * E;
   const int nd0 = tid % NB;
   const TT* __restrict__ Tin = reinterpret_cast<const TT*>(sa.Tin);
   tile_face_table<E>(mesh, e0, sNb, sBt);
@@ -525,4 +553,13 @@ void dgx_tile_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* _
   }
   if (!ok)
     atomicOr(status, 1);
+}
+
+template <int P, int Q, int R, int W, typename TT, bool AFF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
+void dgx_tile_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
+                           int* status)
+{
+  __shared__ __attribute__((aligned(16))) unsigned char lds[TileLds<P, Q, R>::bytes];
+  tile_body_lds<P, Q, R, TT, AFF>(mesh, ph, sa, lam, status, blockIdx.x, lds);
 }

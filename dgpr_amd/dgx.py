@@ -129,6 +129,9 @@ def lib():
         L.dgx_ensemble_autotune.argtypes = [C.c_void_p, C.c_int, PD, C.c_int,
                                             C.POINTER(C.c_int)]
         L.dgx_ensemble_last_error.argtypes = [C.c_void_p]
+        L.dgx_ensemble_set_persistent.argtypes = [C.c_void_p, C.c_int]
+        L.dgx_ensemble_member_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.dgx_ensemble_last_error.restype = C.c_char_p
         L.dgx_sample_output.argtypes = [P_ctx, C.c_int, C.c_int, PD]
         L.dgx_sample_output_device.argtypes = [P_ctx, C.c_int, C.c_int, C.c_void_p]
@@ -164,7 +167,8 @@ def lib():
                    "dgx_read_dg_output",
                    "dgx_ensemble_create", "dgx_ensemble_destroy", "dgx_ensemble_member",
                    "dgx_ensemble_step", "dgx_ensemble_sync", "dgx_ensemble_get_stream",
-                   "dgx_ensemble_autotune", "dgx_kinetic_energy", "dgx_residual_vjp",
+                   "dgx_ensemble_autotune", "dgx_ensemble_set_persistent",
+                   "dgx_ensemble_member_tuning", "dgx_kinetic_energy", "dgx_residual_vjp",
                    "dgx_adjoint_rk4", "dgx_adjoint_release", "dgx_residual_vjp_device_noalloc"):
             getattr(L, nm).restype = st
         _lib = L
@@ -610,6 +614,18 @@ class Ensemble:
         w = (C.c_int * len(self.members))()
         self._chk(lib().dgx_ensemble_autotune(self.h, sc, _p(dts), int(reps), w))
         return list(w)
+
+    def set_persistent(self, on):
+        """True: one persistent launch per stage for the q = 1 members
+        (default); False: one batched launch per (p, q, affine) group."""
+        self._chk(lib().dgx_ensemble_set_persistent(self.h, 1 if on else 0))
+
+    def member_tuning(self, i):
+        """(persistent, tile, waves) of the launch member i runs in."""
+        p, t, w = C.c_int(), C.c_int(), C.c_int()
+        self._chk(lib().dgx_ensemble_member_tuning(self.h, int(i), C.byref(p), C.byref(t),
+                                                   C.byref(w)))
+        return p.value, t.value, w.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -217,12 +217,18 @@ dgx_status dgx_set_mean(dgx_ctx* ctx, const double* mean_host);
  * case per process, dg.cpp / cudg.cu) ----------------------------------------
  * An ensemble owns one context per member (meshes of any size, p and q; own
  * parameters).  dgx_ensemble_step advances every member by nsteps steps of
- * its own dt: per RK stage, the members sharing (p, q) run as one launch of
- * the batched stage kernel (affine q = 1 meshes, whose kernels use one
- * Jacobian per element, form their own groups); with use_graph != 0 one step
- * of the whole ensemble is captured once in a hipGraph and replayed.  Results
- * equal those of dgx_step on each member.  The call only enqueues work on the ensemble's
- * stream: call dgx_ensemble_sync before using a member context again. */
+ * its own dt.  Per RK stage, by default, all q = 1 members of every order run
+ * as ONE launch of a persistent kernel whose workgroups take work units
+ * (rows or tiles of one member's elements, largest members first) from an
+ * atomic counter (affine and curved members form two such launches; q > 1
+ * members run one batched launch per (p, q)); with
+ * dgx_ensemble_set_persistent(ens, 0) every (p, q, affine) group runs its
+ * own batched launch instead.  With use_graph != 0 one step of the whole
+ * ensemble is captured once in a hipGraph and replayed.  Results equal
+ * those of dgx_step on each member (the same arithmetic, to rounding) in
+ * both modes.  The call only
+ * enqueues work on the ensemble's stream: call dgx_ensemble_sync before
+ * using a member context again. */
 typedef struct dgx_ensemble dgx_ensemble;
 dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_params* params,
                                dgx_ensemble** out);
@@ -235,12 +241,22 @@ dgx_status dgx_ensemble_step(dgx_ensemble* ens, int scheme, const double* dt, ui
  * subsonic inflow without a Mach root. */
 dgx_status dgx_ensemble_sync(dgx_ensemble* ens);
 dgx_status dgx_ensemble_get_stream(dgx_ensemble* ens, void** stream);
-/* Times the batched stage kernel of every (p, q) group at each occupancy
- * target (reps launches of the first stage, which writes stage buffers
- * only), keeps the fastest per group and writes each member's choice to
- * waves_out[ncase] (may be NULL). */
+/* Times every launch group of the current mode at each occupancy target and,
+ * for the persistent kernel, each tile (one-row units: 64/(p+1)^3 elements
+ * per workgroup, one node per lane; tiled units: 2, 6 or 8 elements per
+ * workgroup at p = 3, 2, 1, up to 3 nodes per lane), with reps launches of
+ * the first stage (which writes stage buffers only), keeps the fastest per
+ * group and writes each member's occupancy target to waves_out[ncase] (may
+ * be NULL; dgx_ensemble_member_tuning reads the full choice). */
 dgx_status dgx_ensemble_autotune(dgx_ensemble* ens, int scheme, const double* dt, int reps,
                                  int* waves_out);
+/* on != 0 (the default): the persistent single launch per stage for q = 1
+ * members; 0: one batched launch per (p, q, affine) group. */
+dgx_status dgx_ensemble_set_persistent(dgx_ensemble* ens, int on);
+/* The launch member i runs in: persistent (0/1), tile (0/1) and occupancy
+ * target (waves per SIMD). */
+dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* ens, int i, int* persistent, int* tile,
+                                      int* waves);
 const char* dgx_ensemble_last_error(const dgx_ensemble* ens);
 
 /* ---- discrete adjoint (no counterpart in the reference) --------------------

@@ -20,10 +20,13 @@
  *   solver/solve.cpp           residual() on the shared face lists
  *   solver/time_distributed.cpp RK4 / TVD-RK3 / residual norm
  *
- * The reference's own solver cannot be compiled here as an oracle: its
- * residual path calls MPI (solve.cpp:258-387, time_distributed.cpp:33),
- * an external library, so the restatement is pinned instead by the
- * reference tests' known answers and by mathematical invariants (tests/).
+ * The restatement is pinned against the reference's own outputs: its CPU
+ * path compiles from its sources with g++ and the image's MPICH
+ * (oracle/build.py builds oracle/ref_harness.cpp and solver/dg.cpp into
+ * oracle/_ref where the reference is present), and the residuals, states,
+ * norms and driver files it writes are committed as data under
+ * tests/golden/; the reference tests' known answers and mathematical
+ * invariants (tests/) cover the rest.
  *
  * Storage layouts follow the reference exactly:
  *   state U[e][r][bf]  (geometry_common.cpp:467-470)

@@ -34,23 +34,32 @@ def built():
     return True
 
 
-def single_runs(cs, scheme, nsteps):
+def single_runs(cs, scheme, nsteps, tiles=None):
+    """Single-case steps of every member with the stage tile the ensemble
+    runs it with (tiles[i]; by default 0, the one-row kernel of the batched
+    launches and of the persistent kernel before tuning)."""
     out = []
-    for c in cs:
+    for i, c in enumerate(cs):
         with dgx.Solver(c.mesh, c.params) as s:
+            s.tile = tiles[i] if tiles else 0
             s.set_state(c.U0)
             s.step(c.dt, scheme, nsteps, norms=False)
             out.append(s.get_state())
     return out
 
 
-@pytest.mark.parametrize("scheme", ["RK4", "tvdRK3"])
-def test_ensemble_matches_single_cases(built, scheme):
+@pytest.mark.parametrize("scheme,persistent", [("RK4", True), ("tvdRK3", True), ("RK4", False),
+                                               ("tvdRK3", False)])
+def test_ensemble_matches_single_cases(built, scheme, persistent):
     cs = members()
     nsteps = 3
     ref = single_runs(cs, scheme, nsteps)
     dts = [c.dt for c in cs]
     with dgx.Ensemble([c.mesh for c in cs], [c.params for c in cs]) as en:
+        en.set_persistent(persistent)
+        for i, c in enumerate(cs):
+            p, t, w = en.member_tuning(i)
+            assert p == (1 if (persistent and c.mesh.q == 1) else 0)
         for m, c in zip(en.members, cs):
             m.set_state(c.U0)
         en.step(scheme, dts, nsteps, graph=False)
@@ -86,15 +95,50 @@ def test_ensemble_member_matches_oracle(orc, built):
 
 
 def test_ensemble_autotune_keeps_results(built):
+    """The autotuner (occupancy targets; tiles of the persistent kernel)
+    leaves the states alone, and its choice gives the single-case results
+    of the same tile to rounding."""
     cs = members()
-    ref = single_runs(cs, "RK4", 2)
     dts = [c.dt for c in cs]
     with dgx.Ensemble([c.mesh for c in cs], [c.params for c in cs]) as en:
         for m, c in zip(en.members, cs):
             m.set_state(c.U0)
         w = en.autotune("RK4", dts, reps=2)
-        print("  autotuned waves per member", w, flush=True)
+        tun = [en.member_tuning(i) for i in range(len(cs))]
+        print("  autotuned waves per member", w, "tuning", tun, flush=True)
         assert all(x in (1, 2, 3) for x in w)
+        assert all(t[1] in (0, 1) for t in tun)
         en.step("RK4", dts, 2, graph=True)
-        for m, r in zip(en.members, ref):
-            assert rel(m.get_state(), r) < 1e-14
+        got = [m.get_state() for m in en.members]
+    ref = single_runs(cs, "RK4", 2, tiles=[t[1] for t in tun])
+    for g, r in zip(got, ref):
+        assert rel(g, r) < 1e-14
+
+
+def test_member_set_params_between_ensemble_steps(built):
+    """dgx_set_params on a member between ensemble steps: the next steps use
+    the new physics (the item tables and the captured graph are rebuilt),
+    as single-case steps with those parameters do (to rounding), with and
+    without the graph."""
+    cs = members()
+    dts = [c.dt for c in cs]
+    k = 2
+    prm2 = cs[k].params.copy()
+    prm2.mu = 4.0 * cs[k].params.mu
+    with dgx.Solver(cs[k].mesh, cs[k].params) as s:
+        s.tile = 0
+        s.set_state(cs[k].U0)
+        s.step(cs[k].dt, "RK4", 1, norms=False)
+        s.set_params(prm2)
+        s.step(cs[k].dt, "RK4", 2, norms=False)
+        ref = s.get_state()
+    for graph in (False, True):
+        with dgx.Ensemble([c.mesh for c in cs], [c.params for c in cs]) as en:
+            for m, c in zip(en.members, cs):
+                m.set_state(c.U0)
+            en.step("RK4", dts, 1, graph=graph)
+            en.members[k].set_params(prm2)
+            en.step("RK4", dts, 2, graph=graph)
+            got = en.members[k].get_state()
+        print("  graph %s: rel err %.3e" % (graph, rel(got, ref)), flush=True)
+        assert rel(got, ref) < 1e-14

@@ -181,14 +181,17 @@ def test_trace_records_follow_state_and_params(orc, built):
 
 
 def test_occupancy_targets_agree(orc, built):
-    """Every compiled occupancy target of the stage kernel gives the oracle's
-    residual; the autotuner keeps one of them."""
+    """Every compiled occupancy target of the one-row stage kernel gives the
+    oracle's residual; the autotuner keeps one of them (the tiled kernel's
+    targets: test_tiled_kernel_parity)."""
     case = CASES["channel_p3"]()
     m = case.mesh
     geo = orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
     _, f_o = geo.residual(orc_params(orc, case.params), case.U0)
     with dgx.Solver(m, case.params) as s:
         s.set_state(case.U0)
+        assert s.tile == 1 and s.stage_waves == 2  # default at p = 3, q = 1, curved
+        s.tile = 0
         fs = []
         for w in (1, 2, 3):
             s.stage_waves = w
@@ -257,6 +260,7 @@ def test_tiled_kernel_parity(name, orc, built):
     U_o, n_o = geo.step(prm, case.U0, case.dt, case.scheme, case.nsteps)
     with dgx.Solver(m, case.params) as s:
         s.set_state(case.U0)
+        s.tile = 0
         _, f0 = s.residual()
         s.tile = 1
         assert s.tile == 1 and s.stage_waves in (2, 3)
@@ -281,6 +285,7 @@ def test_tile_selection_rules(built):
     it moves a target of 1 to 2; q = 2 reports DGX_ERR_UNSUPPORTED."""
     c = CASES["channel_p2"]()
     with dgx.Solver(c.mesh, c.params) as s:
+        assert s.tile == 0
         s.stage_waves = 1
         s.tile = 1
         assert s.stage_waves == 2

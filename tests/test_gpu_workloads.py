@@ -12,9 +12,10 @@ hold at any size:
   and conserve the total mass to rounding (interior fluxes cancel, the
   no-slip wall's boundary state carries no mass flux, and the residual sums
   to zero against the constant basis function, solve.cpp:484-794);
-* config 3 (the 64-case ensemble): all 64 members built, two batched steps
-  replayed from the hipGraph equal each member's single-case steps bit for
-  bit;
+* config 3 (the 64-case ensemble): all 64 members built, two steps of the
+  persistent launch replayed from the hipGraph equal each member's
+  single-case steps to rounding (1e-13 relative: the same arithmetic in a
+  differently compiled kernel);
 * config 4 (32^3 p = 2 adjoint): the residual dot-product test and the
   gradient of the final kinetic energy after 5 RK4 steps against central
   differences along 3 random directions and in mu.
@@ -103,12 +104,14 @@ def test_config3_full_ensemble_matches_single_cases():
         got = [m.get_state() for m in en.members]
     for i, c in enumerate(cs):
         with dgx.Solver(c.mesh, c.params) as s:
+            s.tile = 0  # the persistent kernel's units before tuning: one row
             s.set_state(c.U0)
             s.step(c.dt, "RK4", 2, norms=False)
             ref = s.get_state()
         assert np.all(np.isfinite(got[i]))
-        assert np.array_equal(got[i], ref), "member %d (n=%d p=%d)" % (i, c.mesh.nx, c.mesh.p)
-    print("  all 64 members bitwise equal to single-case steps", flush=True)
+        e = rel(got[i], ref)
+        assert e < 1e-13, "member %d (n=%d p=%d): %.3e" % (i, c.mesh.nx, c.mesh.p, e)
+    print("  all 64 members equal single-case steps to 1e-13", flush=True)
 
 
 def test_config4_adjoint_full_size():
