@@ -13,10 +13,12 @@ Prints one JSON line:
   roofline   = algorithmic HBM bytes per stage-kernel launch / average launch
                time (HIP events on the library's stream, steps without the
                norm) vs 8 TB/s
-  mixed      = the same workload in the mixed-precision form: throughput,
-               speedup over FP64, and after the workload's 100 RK4 steps
-               from the initial state per-field L2/Linf errors and the
-               kinetic-energy dissipation rate against the FP64 run
+  mixed, mixed_bf16 = the same workload in the two mixed-precision forms
+               (FP32 trace records; volume contractions on the matrix cores
+               in FP32, or in 3-term split BF16): throughput, speedup over
+               FP64, and after the workload's 100 RK4 steps from the initial
+               state per-field L2/Linf errors and the kinetic-energy
+               dissipation rate against the FP64 run
   cpu_baseline = the oracle (CPU restatement of the reference's dense
                residual + RK4) on a bounded sample of the same workload,
                one 8^3 block per host thread
@@ -410,15 +412,26 @@ def field_errors(Um, U64, case, nsteps, k64, eps64, kmx, epsmx):
     }
 
 
-def run_mixed(a, h, s, case, value64, stream, ev0, ev1, norms_dev):
-    """The mixed-precision form on the same workload: the same untimed
-    warmup and tuning and the same timed steps (with per-step norms) from the
-    initial state; then the error leg: the workload's own step count (100
-    RK4 steps for the 64^3 case) from the initial state in both forms,
-    per-field L2/Linf errors of the mixed state against the FP64 state and
-    the kinetic-energy dissipation rate of both over one more step."""
+def fp64_reference(s, case):
+    """The FP64 state after the workload's step count from the initial state
+    and its kinetic energy and dissipation rate (the error legs' reference)."""
+    s.precision = "fp64"
     s.set_state(case.U0)
-    s.precision = "mixed"
+    s.step(case.dt, case.scheme, case.nsteps, norms=False)
+    U64 = s.get_state()
+    k64, eps64 = kinetic_rate(s, case)
+    return U64, k64, eps64
+
+
+def run_mixed(a, h, s, case, value64, stream, ev0, ev1, norms_dev, form, ref):
+    """A mixed-precision form on the same workload: the same untimed warmup
+    and tuning and the same timed steps (with per-step norms) from the
+    initial state; then the error leg: the workload's own step count (100
+    RK4 steps for the 64^3 case) from the initial state, per-field L2/Linf
+    errors of the state against the FP64 run's and the kinetic-energy
+    dissipation rate of both over one more step."""
+    s.set_state(case.U0)
+    s.precision = form
     s.step_async(case.dt, case.scheme, a.warmup)
     s.sync()
     if a.autotune:
@@ -429,17 +442,13 @@ def run_mixed(a, h, s, case, value64, stream, ev0, ev1, norms_dev):
     gpu_s = timed_steps(h, s, case, a.steps, stream, ev0, ev1, norms_dev)
     value = case.dof * case.stages * a.steps / gpu_s / 1e9
     gpu_s_nn = timed_steps(h, s, case, a.steps, stream, ev0, ev1)
-    # error leg
     nerr = case.nsteps
     s.set_state(case.U0)
     s.step(case.dt, case.scheme, nerr, norms=False)
     Um = s.get_state()
     kmx, epsmx = kinetic_rate(s, case)
     s.precision = "fp64"
-    s.set_state(case.U0)
-    s.step(case.dt, case.scheme, nerr, norms=False)
-    U64 = s.get_state()
-    k64, eps64 = kinetic_rate(s, case)
+    U64, k64, eps64 = ref
     return {
         "value": value,
         "unit": "GDOF-stage/s",
@@ -448,7 +457,7 @@ def run_mixed(a, h, s, case, value64, stream, ev0, ev1, norms_dev):
         "avg_launch_us": gpu_s_nn / (a.steps * case.stages) * 1e6,
         "stage_tile": tile,
         "stage_waves_per_simd": waves,
-        "form": s.MIXED_FORM,
+        "form": s.MIXED_FORMS[form],
         "error_vs_fp64": field_errors(Um, U64, case, nerr, k64, eps64, kmx, epsmx),
     }
 
@@ -531,7 +540,11 @@ def main():
         h.hipFree(dst)
 
     kernel_name = s.stage_kernel_name(waves, tile)
-    mixed = None if a.no_mixed else run_mixed(a, h, s, case, value, stream, ev0, ev1, norms_dev)
+    mixed = {}
+    if not a.no_mixed:
+        ref = fp64_reference(s, case)
+        for form in ("mixed", "mixed_bf16"):
+            mixed[form] = run_mixed(a, h, s, case, value, stream, ev0, ev1, norms_dev, form, ref)
     h.hipFree(norms_dev)
     s.close()
 
@@ -605,8 +618,7 @@ def main():
             "flops_per_dof_stage": flops / case.dof,
         },
     }
-    if mixed is not None:
-        out["mixed"] = mixed
+    out.update(mixed)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds, a.cpu_threads)
     print(json.dumps(out), flush=True)

@@ -3,8 +3,9 @@
 //
 // Everything is the exact transpose of what the device computes (the same
 // Gauss-nodal residual f_g, the stage algebra of stage_plan), so the
-// gradients are those of the discrete forward map.  Periodic meshes only:
-// the reversed face kernels assume every face has a neighbour.
+// gradients are those of the discrete forward map, on meshes with any
+// boundary conditions (domain-boundary faces are reversed through their
+// boundary states by a kernel of their own).
 //
 // RK4 step (time_distributed.cpp:46-76): S1 = V + dt/2 k1, S2 = V + dt/2 k2,
 // S3 = V + dt k3, V' = V + dt/6 (k1 + 2 k2 + 2 k3 + k4), k_i = f(S_(i-1)).
@@ -68,14 +69,6 @@ dgx_status afail(dgx_ctx* c, dgx_status st, const char* fmt, ...)
       return afail((c), DGX_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));       \
   } while (0)
 
-bool all_periodic(const dgx_ctx* c)
-{
-  for (int i = 0; i < 6; ++i)
-    if (c->bounds[i] != DGX_BT_PERIODIC)
-      return false;
-  return true;
-}
-
 // Workspace of the residual adjoint
 struct AdjScratch
 {
@@ -116,6 +109,7 @@ hipError_t stage(dgx_ctx* c, const Phys& ph, const double* Vin, const double* Ti
   sa.cf = cf;
   sa.accmode = 0;
   sa.trace32 = 0;
+  sa.ctype   = 0;
   return rhs_dispatch(c->p, c->q, mesh_args(c), ph, sa, c->lam, c->status, c->stage_waves,
                       c->stream, c->stage_tile);
 }
@@ -199,8 +193,6 @@ dgx_status dgx_residual_vjp(dgx_ctx* c, const double* w_host, double* g_host)
 {
   if (!c || !w_host || !g_host)
     return DGX_ERR_INVALID;
-  if (!all_periodic(c))
-    return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs a periodic mesh");
   if (c->prec != DGX_PREC_FP64)
     return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs DGX_PREC_FP64");
   AdjScratch s;
@@ -226,8 +218,6 @@ dgx_status dgx_residual_vjp_device_noalloc(dgx_ctx* c, const double* w_dev, doub
 {
   if (!c || !w_dev || !g_dev)
     return DGX_ERR_INVALID;
-  if (!all_periodic(c))
-    return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs a periodic mesh");
   if (c->prec != DGX_PREC_FP64)
     return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs DGX_PREC_FP64");
   const size_t n = c->nstate;
@@ -273,8 +263,6 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
 {
   if (!c || nsteps == 0 || !(dt > 0.0))
     return DGX_ERR_INVALID;
-  if (!all_periodic(c))
-    return afail(c, DGX_ERR_UNSUPPORTED, "the discrete adjoint needs a periodic mesh");
   if (c->prec != DGX_PREC_FP64)
     return afail(c, DGX_ERR_UNSUPPORTED, "the discrete adjoint needs DGX_PREC_FP64");
   if (dJdmu && !(c->phys.mu > 0.0))
@@ -426,6 +414,7 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
         sa.cf = cstage[i];
         sa.accmode = accmode[i];
         sa.trace32 = 0;
+        sa.ctype   = 0;
         AHIP(c, rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status,
                              c->stage_waves, c->stream, c->stage_tile));
       }

@@ -2,8 +2,9 @@
 // compiled in the translation unit of dgx_kernels.hip (they share its
 // constant tables and helpers; included from there inside namespace dgx).
 //
-// Given psi, the adjoint of the Gauss-nodal right-hand side f_g(V), the two
-// kernels compute psi_V = (d f_g / d V)^T psi for periodic meshes.  The
+// Given psi, the adjoint of the Gauss-nodal right-hand side f_g(V), the
+// kernels compute psi_V = (d f_g / d V)^T psi (every boundary type:
+// domain-boundary face points are finished by dgx_adj_boundary_kernel).  The
 // forward evaluation of one element (volume terms, its six faces evaluated
 // from its own side with both traces from the trace records, gather,
 // f_g = s - acc / wJ) is reversed step by step:
@@ -232,22 +233,35 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
         ph1[k] = -s1;
         ph2[k] = -s2;
       }
-      // forward quantities of this face point
+      // forward quantities of this face point; on a domain-boundary face UR
+      // is the boundary state stored with the records
       const int en = sNb[l][f];
+      const bool interior = en >= 0;
       const double* To = Tin + ((size_t)ee * 6 + f) * 8 * NF + fpt;
-      const double* Tn = Tin + ((size_t)en * 6 + (f ^ 1)) * 8 * NF + fpt;
-      double UL[5], UR[5], vgL[3], vgR[3];
+      double UL[5], UR[5], vgL[3], vgR[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int r = 0; r < 5; ++r)
-      {
         UL[r] = To[r * NF];
-        UR[r] = Tn[r * NF];
-      }
 #pragma unroll
       for (int i = 0; i < 3; ++i)
-      {
         vgL[i] = To[(5 + i) * NF];
-        vgR[i] = -Tn[(5 + i) * NF];
+      if (interior)
+      {
+        const double* Tn = Tin + ((size_t)en * 6 + (f ^ 1)) * 8 * NF + fpt;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          UR[r] = Tn[r * NF];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          vgR[i] = -Tn[(5 + i) * NF];
+      }
+      else
+      {
+        const double* B =
+            Tin + (size_t)mesh.nelem * 6 * 8 * NF + (size_t)(-1 - en) * 5 * NF + fpt;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          UR[r] = B[r * NF];
       }
       const int gi0 = (nd == 0) ? 0 : a;
       const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
@@ -257,9 +271,12 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
       double nv[3];
       const double fJ = face_normal(J, nd, hi, nv);
       const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
-      const double lamR = lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
+      const double lamR = interior ? lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt] : 0.0;
       double S[6];
-      interior_dc_stress(UL, UR, nv, ph.mu, S);
+      if (interior)
+        interior_dc_stress(UL, UR, nv, ph.mu, S);
+      else
+        boundary_dc_stress(UL, UR, nv, ph.mu, S);
       const double irL = rcp(UL[0]);
       const double uL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
       const double sc = fJ * pt.w[a] * pt.w[b];
@@ -302,6 +319,31 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
 #pragma unroll
       for (int r = 0; r < 5; ++r)
         pG[r] = sc * pT[r];
+      if (!interior)
+      {
+        // boundary face: the adjoints of G (into the boundary section of
+        // anbr) and S (into the element's own anbr slot) go to
+        // dgx_adj_boundary_kernel, which reverses the boundary physics; the
+        // u_L part of the contravariant energy flux is the trace's already
+        double* An = anbr + ((size_t)ee * 6 + f) * 8 * NF + fpt;
+        double* Bg = anbr + (size_t)mesh.nelem * 6 * 8 * NF + (size_t)(-1 - en) * 5 * NF + fpt;
+        double* Ao = aown + ((size_t)ee * 6 + f) * 8 * NF + fpt;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          Bg[r * NF] = pG[r];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+          An[r * NF] = pS[r];
+        Ao[0] = -(puL[0] * uL[0] + puL[1] * uL[1] + puL[2] * uL[2]) * irL;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          Ao[(1 + i) * NF] = puL[i] * irL;
+#pragma unroll
+        for (int r = 4; r < 8; ++r)
+          Ao[r * NF] = 0.0;
+      }
+      else
+      {
       double gx[16];
       face_physics_vjp(UL, UR, vgL, vgR, nv, lamL, lamR, ph, pG, pS, gx);
       // u_L = m_L / rho_L
@@ -323,9 +365,132 @@ __global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph
         Ao[(5 + i) * NF] = gx[10 + i];
         An[(5 + i) * NF] = -gx[13 + i];  // the neighbour stored its traction with -n
       }
+      }
     }
     __syncthreads();
   }
+}
+
+// Boundary faces (walls, inflow, outflow, free stream), one thread per
+// boundary face point.  The face-term kernel left there the adjoint of the
+// face flux G (5, in the boundary section of anbr) and of the
+// dual-consistency stress S (6, in the element's own anbr slot), and wrote
+// into aown the part of the trace adjoint that passes through u_L in the
+// contravariant energy flux.  This kernel reverses the boundary face physics
+// (face_point_physics(false, ...)), including the boundary state's
+// dependence on the trace (boundary_state, bound_states.cpp:24-130), with
+// forward-mode dual numbers, one pass per trace input (5 state, 3 traction),
+// adds the result to aown and clears the element's anbr slot (a boundary
+// face has no neighbour adjoint).
+template <int P, int Q>
+__global__ __launch_bounds__(64) void dgx_adj_boundary_kernel(MeshArgs mesh, Phys ph,
+                                                              const double* __restrict__ Tin,
+                                                              const double* __restrict__ lam,
+                                                              double* __restrict__ aown,
+                                                              double* __restrict__ anbr)
+{
+  constexpr int N = P + 1, NF = N * N, NG = (Q + 1) * (Q + 1) * (Q + 1);
+  const int nx = mesh.nx, ny = mesh.ny, nz = mesh.nz;
+  const int sx = ny * nz, sy = nx * nz, sz = nx * ny;
+  const size_t t = (size_t)blockIdx.x * 64 + threadIdx.x;
+  if (t >= (size_t)2 * (sx + sy + sz) * NF)
+    return;
+  const int bf = (int)(t / NF), fpt = (int)(t % NF);
+  // side f and element of boundary face bf (the inverse of bface_index)
+  int f, in;
+  if (bf < 2 * sx)
+  {
+    f  = bf / sx;
+    in = bf % sx;
+  }
+  else if (bf < 2 * sx + 2 * sy)
+  {
+    f  = 2 + (bf - 2 * sx) / sy;
+    in = (bf - 2 * sx) % sy;
+  }
+  else
+  {
+    f  = 4 + (bf - 2 * sx - 2 * sy) / sz;
+    in = (bf - 2 * sx - 2 * sy) % sz;
+  }
+  const int bt = mesh.bounds[f];
+  if (bt == -2)
+    return;  // periodic side: no boundary faces
+  const int nd = f >> 1, hi = f & 1;
+  int ci, cj, ck;
+  if (nd == 0)
+  {
+    ci = hi ? nx - 1 : 0;
+    cj = in % ny;
+    ck = in / ny;
+  }
+  else if (nd == 1)
+  {
+    cj = hi ? ny - 1 : 0;
+    ci = in % nx;
+    ck = in / nx;
+  }
+  else
+  {
+    ck = hi ? nz - 1 : 0;
+    ci = in % nx;
+    cj = in / nx;
+  }
+  const int e = (ck * ny + cj) * nx + ci;
+  const int a = fpt % N, b = fpt / N;
+  const int gi0 = (nd == 0) ? 0 : a;
+  const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
+  const int gi2 = (nd == 2) ? 0 : b;
+  double J[3][3];
+  jac_at<Q>(c_qtab[P - 1][Q - 1], mesh.nodes + (size_t)e * NG * 3, nd, hi, gi0, gi1, gi2, J);
+  double nv[3];
+  face_normal(J, nd, hi, nv);
+  const double* To = Tin + ((size_t)e * 6 + f) * 8 * NF + fpt;
+  double* Ao = aown + ((size_t)e * 6 + f) * 8 * NF + fpt;
+  double* An = anbr + ((size_t)e * 6 + f) * 8 * NF + fpt;
+  const double* Bg = anbr + (size_t)mesh.nelem * 6 * 8 * NF + (size_t)bf * 5 * NF + fpt;
+  const double lamL = lam[((size_t)e * 6 + f) * NF + fpt];
+  double x[8], pG[5], pS[6];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+    x[r] = To[r * NF];
+#pragma unroll
+  for (int r = 0; r < 5; ++r)
+    pG[r] = Bg[r * NF];
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+    pS[r] = An[r * NF];
+  for (int d = 0; d < 8; ++d)
+  {
+    Dual<1> UL[5], vgL[3], vgR[3], Ub[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+    {
+      UL[r]      = Dual<1>(x[r]);
+      UL[r].d[0] = (d == r) ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+      vgL[i]      = Dual<1>(x[5 + i]);
+      vgL[i].d[0] = (d == 5 + i) ? 1.0 : 0.0;
+      vgR[i]      = Dual<1>(0.0);
+    }
+    (void)boundary_state(bt, UL, nv, ph, Ub);
+    FacePointT<Dual<1>> o;
+    face_point_physics(false, UL, Ub, vgL, vgR, nv, lamL, 0.0, ph, o);
+    double g = 0.0;
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+      g += pG[r] * o.G[r].d[0];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+      g += pS[r] * o.S[r].d[0];
+    Ao[d * NF] += g;
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+    An[r * NF] = 0.0;
 }
 
 template <int P, int Q, bool AFF>

@@ -414,7 +414,8 @@ static hipError_t rhs(dgx_ctx* c, const double* Vin, const double* Tin, const do
   sa.cb = cb;
   sa.cf = cf;
   sa.accmode = accmode;
-  sa.trace32 = c->prec == DGX_PREC_MIXED;
+  sa.trace32 = c->prec != DGX_PREC_FP64;
+  sa.ctype   = c->prec == DGX_PREC_MIXED ? 1 : (c->prec == DGX_PREC_MIXED_BF16 ? 2 : 0);
   return rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status, c->stage_waves,
                       c->stream, c->stage_tile);
 }
@@ -427,7 +428,7 @@ hipError_t dgx::ensure_traces(dgx_ctx* c)
   if (c->tu_valid)
     return hipSuccess;
   hipError_t e = trace_dispatch(c->p, c->q, mesh_args(c), c->phys, c->U, c->TU, c->status,
-                                c->stream, c->prec == DGX_PREC_MIXED);
+                                c->stream, c->prec != DGX_PREC_FP64);
   if (e == hipSuccess)
     c->tu_valid = true;
   return e;
@@ -454,7 +455,7 @@ dgx_status dgx_residual_device(dgx_ctx* c, const double* U_dev, double* R_dev, d
     return DGX_ERR_INVALID;
   HIPCHK(c, basis_dispatch(c->p, U_dev, c->Ua, c->nelem, 0, c->stream));
   HIPCHK(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys, c->Ua, c->TUa, c->status,
-                           c->stream, c->prec == DGX_PREC_MIXED));
+                           c->stream, c->prec != DGX_PREC_FP64));
   HIPCHK(c, rhs(c, c->Ua, c->TUa, nullptr, nullptr, nullptr, nullptr, c->fs, 0, 0, 0, 0));
   HIPCHK(c, resid_out_dispatch(c->p, c->q, mesh_args(c), c->fs, f_dev, R_dev, nullptr,
                                c->stream));
@@ -484,8 +485,9 @@ dgx_status dgx_residual(dgx_ctx* c, double* R_host, double* f_host)
 // instead of recomputing.
 int dgx::stage_plan(const dgx_ctx* c, int scheme, double dt, StageArgs sa[4])
 {
-  const int t32 = c->prec == DGX_PREC_MIXED;
-  auto set = [t32](StageArgs& a, const double* Vin, const double* Tin, const double* Vbase,
+  const int t32 = c->prec != DGX_PREC_FP64;
+  const int ct  = c->prec == DGX_PREC_MIXED ? 1 : (c->prec == DGX_PREC_MIXED_BF16 ? 2 : 0);
+  auto set = [t32, ct](StageArgs& a, const double* Vin, const double* Tin, const double* Vbase,
                    double* acc, double* Vout, double* Tout, double ca, double cb, double cf,
                    int accmode) {
     a.Vin = Vin;
@@ -500,6 +502,7 @@ int dgx::stage_plan(const dgx_ctx* c, int scheme, double dt, StageArgs sa[4])
     a.cf = cf;
     a.accmode = accmode;
     a.trace32 = t32;
+    a.ctype   = ct;
   };
   if (scheme == DGX_RK4)
   {
@@ -608,7 +611,8 @@ dgx_status dgx_step(dgx_ctx* c, int scheme, double dt, uint64_t nsteps, double* 
 
 dgx_status dgx_set_precision(dgx_ctx* c, int precision)
 {
-  if (!c || (precision != DGX_PREC_FP64 && precision != DGX_PREC_MIXED))
+  if (!c || (precision != DGX_PREC_FP64 && precision != DGX_PREC_MIXED &&
+             precision != DGX_PREC_MIXED_BF16))
     return DGX_ERR_INVALID;
   if (precision != c->prec)
   {

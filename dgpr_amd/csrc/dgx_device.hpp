@@ -381,9 +381,12 @@ __device__ __forceinline__ void visc_apply_n(const ViscState& vs, const double G
 
 // Boundary state Ub from interior trace UL and outward normal n.
 // Returns false when the subsonic-inflow Mach quadratic has no
-// non-negative root (the reference aborts there on the host).
-__device__ __forceinline__ bool boundary_state(int bt, const double* UL, const double* n,
-                                               const Phys& ph, double* Ub)
+// non-negative root (the reference aborts there on the host).  T is double
+// in the residual kernels and a dual number in the adjoint's boundary-face
+// kernel (the same expressions, so the double instantiation is unchanged).
+template <typename T>
+__device__ __forceinline__ bool boundary_state(int bt, const T* UL, const double* n,
+                                               const Phys& ph, T* Ub)
 {
   double gamma = ph.gamma, gm1 = gamma - 1.0;
   switch (bt)
@@ -395,9 +398,9 @@ __device__ __forceinline__ bool boundary_state(int bt, const double* UL, const d
       return true;
     case -4:  // inviscid wall: remove the normal velocity
     {
-      double ir = rcp(UL[0]);
-      double up = UL[1] * ir, vp = UL[2] * ir, wp = UL[3] * ir;
-      double vdn = up * n[0] + vp * n[1] + wp * n[2];
+      T ir = rcp(UL[0]);
+      T up = UL[1] * ir, vp = UL[2] * ir, wp = UL[3] * ir;
+      T vdn = up * n[0] + vp * n[1] + wp * n[2];
       Ub[0] = UL[0];
       Ub[1] = UL[0] * (up - vdn * n[0]);
       Ub[2] = UL[0] * (vp - vdn * n[1]);
@@ -411,20 +414,20 @@ __device__ __forceinline__ bool boundary_state(int bt, const double* UL, const d
     case -5:  // subsonic inflow: total T, total P, outgoing Riemann invariant
     {
       double R = ph.Rgas, Tt = ph.Tt, Pt = ph.Pt;
-      double ir = rcp(UL[0]);
-      double up = UL[1] * ir, vp = UL[2] * ir, wp = UL[3] * ir;
-      double Vsq = up * up + vp * vp + wp * wp;
-      double pp  = gm1 * (UL[4] - 0.5 * UL[0] * Vsq);
-      double unp = up * n[0] + vp * n[1] + wp * n[2];
-      double cp  = sqrt(gamma * pp * ir);
-      double Jp  = unp + ((2.0 * cp) / gm1);
-      double a   = gamma * R * Tt - 0.5 * gm1 * Jp * Jp;
-      double b   = (4.0 * gamma * R * Tt * -1.0) / gm1;
-      double c   = (4.0 * gamma * R * Tt) / (gm1 * gm1) - Jp * Jp;
-      double disc = sqrt(b * b - 4.0 * a * c);
-      double s0  = (-b + disc) / (2.0 * a);
-      double s1  = (-b - disc) / (2.0 * a);
-      double Mb  = 0.0;
+      T ir = rcp(UL[0]);
+      T up = UL[1] * ir, vp = UL[2] * ir, wp = UL[3] * ir;
+      T Vsq = up * up + vp * vp + wp * wp;
+      T pp  = gm1 * (UL[4] - 0.5 * UL[0] * Vsq);
+      T unp = up * n[0] + vp * n[1] + wp * n[2];
+      T cp  = sqrt(gamma * pp * ir);
+      T Jp  = unp + ((2.0 * cp) / gm1);
+      T a   = gamma * R * Tt - 0.5 * gm1 * Jp * Jp;
+      double b = (4.0 * gamma * R * Tt * -1.0) / gm1;
+      T c   = (4.0 * gamma * R * Tt) / (gm1 * gm1) - Jp * Jp;
+      T disc = sqrt(b * b - 4.0 * a * c);
+      T s0  = (-b + disc) / (2.0 * a);
+      T s1  = (-b - disc) / (2.0 * a);
+      T Mb  = 0.0;
       bool ok    = true;
       if (s0 >= 0.0 && s1 >= 0.0)
         Mb = (s0 < s1) ? s0 : s1;
@@ -434,11 +437,11 @@ __device__ __forceinline__ bool boundary_state(int bt, const double* UL, const d
         Mb = s1;
       else
         ok = false;
-      double Tb   = Tt / (1.0 + 0.5 * gm1 * Mb * Mb);
-      double Pb   = Pt * pow(Tb / Tt, gamma / gm1);
-      double rhob = Pb / (R * Tb);
-      double cb   = sqrt((gamma * Pb) / rhob);
-      double ub = Mb * cb * -n[0], vb = Mb * cb * -n[1], wb = Mb * cb * -n[2];
+      T Tb   = Tt / (1.0 + 0.5 * gm1 * Mb * Mb);
+      T Pb   = Pt * pow(Tb / Tt, gamma / gm1);
+      T rhob = Pb / (R * Tb);
+      T cb   = sqrt((gamma * Pb) / rhob);
+      T ub = Mb * cb * -n[0], vb = Mb * cb * -n[1], wb = Mb * cb * -n[2];
       Ub[0] = rhob; Ub[1] = rhob * ub; Ub[2] = rhob * vb; Ub[3] = rhob * wb;
       Ub[4] = (Pb / gm1) + 0.5 * rhob * (ub * ub + vb * vb + wb * wb);
       return ok;
@@ -446,21 +449,21 @@ __device__ __forceinline__ bool boundary_state(int bt, const double* UL, const d
     case -6:  // subsonic outflow: fixed static pressure, interior entropy
     {
       double P   = ph.P;
-      double rp  = UL[0];
-      double ir  = rcp(rp);
-      double up = UL[1] * ir, vp = UL[2] * ir, wp = UL[3] * ir;
-      double Vsq = up * up + vp * vp + wp * wp;
-      double pp  = gm1 * (UL[4] - 0.5 * rp * Vsq);
-      double Sp  = pp / pow(rp, gamma);
-      double rhob = pow(P / Sp, 1.0 / gamma);
-      double cb   = sqrt((gamma * P) / rhob);
-      double unp  = up * n[0] + vp * n[1] + wp * n[2];
-      double cp   = sqrt(gamma * pp * ir);
-      double Jp   = unp + ((2.0 * cp) / gm1);
-      double unb  = Jp - ((2.0 * cb) / gm1);
-      double ub = up - unp * n[0] + unb * n[0];
-      double vb = vp - unp * n[1] + unb * n[1];
-      double wb = wp - unp * n[2] + unb * n[2];
+      T rp  = UL[0];
+      T ir  = rcp(rp);
+      T up = UL[1] * ir, vp = UL[2] * ir, wp = UL[3] * ir;
+      T Vsq = up * up + vp * vp + wp * wp;
+      T pp  = gm1 * (UL[4] - 0.5 * rp * Vsq);
+      T Sp  = pp / pow(rp, gamma);
+      T rhob = pow(P / Sp, 1.0 / gamma);
+      T cb   = sqrt((gamma * P) / rhob);
+      T unp  = up * n[0] + vp * n[1] + wp * n[2];
+      T cp   = sqrt(gamma * pp * ir);
+      T Jp   = unp + ((2.0 * cp) / gm1);
+      T unb  = Jp - ((2.0 * cb) / gm1);
+      T ub = up - unp * n[0] + unb * n[0];
+      T vb = vp - unp * n[1] + unb * n[1];
+      T wb = wp - unp * n[2] + unb * n[2];
       Ub[0] = rhob; Ub[1] = rhob * ub; Ub[2] = rhob * vb; Ub[3] = rhob * wb;
       Ub[4] = (P / gm1) + 0.5 * rhob * (ub * ub + vb * vb + wb * wb);
       return true;

@@ -3,7 +3,9 @@
 // generic scalar type, so running it on Dual<D> inputs seeded with unit
 // directions gives D columns of its Jacobian at once; the reverse sweeps of
 // the discrete adjoint (dgx_vjp.hpp, roe_flux_vjp) are checked against it
-// (tests/cpu/vjp_check.cpp).  Not used by the kernels.
+// (tests/cpu/vjp_check.cpp), and the adjoint's boundary-face kernel (a few
+// face points per mesh) differentiates the boundary states and the boundary
+// face physics with it.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,6 +17,7 @@ namespace dgx
 
 // the dual overloads below must not hide the double ones
 using ::fabs;
+using ::pow;
 using ::sqrt;
 
 template <int D>
@@ -160,6 +163,53 @@ template <int D>
 __device__ __forceinline__ Dual<D> fabs(const Dual<D>& x)
 {
   return (x.v < 0.0) ? -x : x;
+}
+template <int D>
+__device__ __forceinline__ bool operator>=(const Dual<D>& a, double b)
+{
+  return a.v >= b;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator/(const Dual<D>& a, const Dual<D>& b)
+{
+  Dual<D> r;
+  r.v = a.v / b.v;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator/(const Dual<D>& a, double b)
+{
+  Dual<D> r;
+  r.v = a.v / b;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = a.d[k] / b;
+  return r;
+}
+template <int D>
+__device__ __forceinline__ Dual<D> operator/(double a, const Dual<D>& b)
+{
+  Dual<D> r;
+  r.v = a / b.v;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = -r.v * b.d[k] / b.v;
+  return r;
+}
+// x^e for x > 0
+template <int D>
+__device__ __forceinline__ Dual<D> pow(const Dual<D>& x, double e)
+{
+  Dual<D> r;
+  r.v = ::pow(x.v, e);
+  const double s = e * ::pow(x.v, e - 1.0);
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    r.d[k] = s * x.d[k];
+  return r;
 }
 
 }  // namespace dgx

@@ -3,6 +3,9 @@
 // adjoint, with their host launch helpers.  The device code shared with the
 // persistent ensemble kernel lives in dgx_stage.hpp (method described there).
 #include "dgx_stage.hpp"
+#include "dgx_dual.hpp"
+
+#include <type_traits>
 
 namespace dgx
 {
@@ -438,22 +441,37 @@ __global__ __launch_bounds__(256) void dgx_mean_kernel(const double* __restrict_
 
 // ---- launch helpers (host) -----------------------------------------------
 
-template <int P, int Q, typename TT, bool AFF>
-hipError_t launch_tile_t(const MeshArgs& m, const Phys& ph, const StageArgs& sa,
-                         const double* lam, int* status, int waves, hipStream_t st)
+template <int P, int Q, typename TT, bool AFF, int CT>
+hipError_t launch_tile_ct(const MeshArgs& m, const Phys& ph, const StageArgs& sa,
+                          const double* lam, int* status, int waves, hipStream_t st)
 {
   constexpr int R = tile_rows(P);
   constexpr int E = TileDims<P, R>::E;
   int nblk = (m.nelem + E - 1) / E;
   if (waves == 2)
-    hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 2, TT, AFF>), dim3(nblk), dim3(64), 0, st,
-                       m, ph, sa, lam, status);
+    hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 2, TT, AFF, CT>), dim3(nblk), dim3(64), 0,
+                       st, m, ph, sa, lam, status);
   else if (waves == 3)
-    hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 3, TT, AFF>), dim3(nblk), dim3(64), 0, st,
-                       m, ph, sa, lam, status);
+    hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 3, TT, AFF, CT>), dim3(nblk), dim3(64), 0,
+                       st, m, ph, sa, lam, status);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
+}
+
+// the matrix-core contraction forms exist for p = 3 with FP32 trace records
+template <int P, int Q, typename TT, bool AFF>
+hipError_t launch_tile_t(const MeshArgs& m, const Phys& ph, const StageArgs& sa,
+                         const double* lam, int* status, int waves, hipStream_t st)
+{
+  if constexpr (P == 3 && std::is_same<TT, float>::value)
+  {
+    if (sa.ctype == 1)
+      return launch_tile_ct<P, Q, TT, AFF, 1>(m, ph, sa, lam, status, waves, st);
+    if (sa.ctype == 2)
+      return launch_tile_ct<P, Q, TT, AFF, 2>(m, ph, sa, lam, status, waves, st);
+  }
+  return launch_tile_ct<P, Q, TT, AFF, 0>(m, ph, sa, lam, status, waves, st);
 }
 
 template <int P, int Q, typename TT, bool AFF>
@@ -726,6 +744,18 @@ hipError_t launch_adjoint_t(const MeshArgs& m, const Phys& ph, const double* V, 
   e = hipGetLastError();
   if (e != hipSuccess)
     return e;
+  bool walls = false;
+  for (int i = 0; i < 6; ++i)
+    walls = walls || m.bounds[i] != -2;
+  if (walls)
+  {
+    const size_t npts = boundary_faces(m.nx, m.ny, m.nz) * (size_t)((P + 1) * (P + 1));
+    hipLaunchKernelGGL((dgx_adj_boundary_kernel<P, Q>), dim3((unsigned)((npts + 63) / 64)),
+                       dim3(64), 0, st, m, ph, Tin, lam, aown, anbr);
+    e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
+  }
   hipLaunchKernelGGL((dgx_adj_trace_kernel<P, Q, AFF>), dim3(nblk), dim3(64), 0, st, m, ph, V,
                      aown, anbr, pvol, psiV);
   return hipGetLastError();

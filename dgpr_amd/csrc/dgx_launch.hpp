@@ -75,6 +75,8 @@ struct StageArgs
   double ca, cb, cf;
   int accmode;
   int trace32;
+  int ctype;  // volume contractions of the tiled p = 3 kernel: 0 FP64, 1 FP32
+              // matrix cores, 2 split-BF16 matrix cores (with trace32 set)
 };
 
 // One member of a batched stage launch: its mesh, physics, stage arguments

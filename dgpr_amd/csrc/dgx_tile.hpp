@@ -22,6 +22,8 @@
 // helpers.
 #pragma once
 
+#include "dgx_mfma.hpp"
+
 // rows per tile of the tiled stage kernel at each order (0: none compiled):
 // p = 3: two one-element rows, so each face subpass is the 64 points of one
 // direction of two elements; p = 1: one row of 8 elements (16 faces of 4
@@ -367,12 +369,16 @@ struct TileLds
 // state (see stage_body_lds for the meaning of every argument; the
 // arithmetic per point is the same), in the LDS region lds
 // (TileLds<P, Q, R>::bytes, 16-byte aligned).
-template <int P, int Q, int R, typename TT, bool AFF>
+// CT selects how the volume contractions are done: 0 in FP64 on the vector
+// ALUs; at p = 3, 1 in FP32 and 2 in split BF16 on the matrix cores
+// (dgx_mfma.hpp), the flux physics between them staying FP64.
+template <int P, int Q, int R, typename TT, bool AFF, int CT = 0>
 __device__ __forceinline__ void tile_body_lds(const MeshArgs& mesh, const Phys& ph,
                                               const StageArgs& sa,
                                               const double* __restrict__ lam, int* status,
                                               int block, unsigned char* lds)
 {
+  static_assert(CT == 0 || P == 3, "matrix-core contractions need p = 3 (4-point lines)");
   using TD = TileDims<P, R>;
   using TL = TileLds<P, Q, R>;
   constexpr int N = TD::N, NB = TD::NB, EPR = TD::EPR, E = TD::E;
@@ -434,6 +440,44 @@ __device__ __forceinline__ void tile_body_lds(const MeshArgs& mesh, const Phys& 
     wJ[rr] = 1.0;
     double Ji[3][3];
     double H[3][5];
+    if constexpr (CT != 0)
+    {
+      // every lane takes part in the matrix-core contractions
+      double gr[3][4];
+      mfma_grad<0, CT>(sV[le], pt, gr[0]);
+      mfma_grad<1, CT>(sV[le], pt, gr[1]);
+      mfma_grad<2, CT>(sV[le], pt, gr[2]);
+      if (vact)
+      {
+        double J[3][3], det;
+        geometry_at<Q, AFF>(qt, sX[le], sG[le], -1, 0, idx[0], idx[1], idx[2], J, Ji, det);
+        wJ[rr] = pt.w[idx[0]] * pt.w[idx[1]] * pt.w[idx[2]] * det;
+        double Vq[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          Vq[r] = sV[le][r][nd0];
+        volume_flux(Vq, gr, Ji, ph.gamma, ph.mu, H);
+      }
+      auto sF = reinterpret_cast<float(*)[64]>(sR);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+      {
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          sF[r][tid] = vact ? (float)(wJ[rr] * (Ji[c][0] * H[0][r] + Ji[c][1] * H[1][r] +
+                                                 Ji[c][2] * H[2][r]))
+                            : 0.0f;
+        __syncthreads();
+        if (c == 0)
+          mfma_project<0, CT>(sF, pt, acc[rr]);
+        else if (c == 1)
+          mfma_project<1, CT>(sF, pt, acc[rr]);
+        else
+          mfma_project<2, CT>(sF, pt, acc[rr]);
+        __syncthreads();
+      }
+      continue;
+    }
     if (vact)
     {
       double J[3][3], det;
@@ -555,11 +599,11 @@ __device__ __forceinline__ void tile_body_lds(const MeshArgs& mesh, const Phys& 
     atomicOr(status, 1);
 }
 
-template <int P, int Q, int R, int W, typename TT, bool AFF>
+template <int P, int Q, int R, int W, typename TT, bool AFF, int CT = 0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 void dgx_tile_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
                            int* status)
 {
   __shared__ __attribute__((aligned(16))) unsigned char lds[TileLds<P, Q, R>::bytes];
-  tile_body_lds<P, Q, R, TT, AFF>(mesh, ph, sa, lam, status, blockIdx.x, lds);
+  tile_body_lds<P, Q, R, TT, AFF, CT>(mesh, ph, sa, lam, status, blockIdx.x, lds);
 }

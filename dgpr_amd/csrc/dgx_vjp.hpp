@@ -33,6 +33,27 @@ __device__ __forceinline__ void interior_dc_stress(const double* UL, const doubl
   S[5] = mu * (a[1] * n[2] + a[2] * n[1]);
 }
 
+// The dual-consistency stress S of a boundary face point, the out.S of
+// face_point_physics(false, ...): tau(dm (x) n) with dm = (m jump - u_L rho
+// jump)/rho_L and the jump UL - Ub to the boundary state (no 1/2).
+__device__ __forceinline__ void boundary_dc_stress(const double* UL, const double* Ub,
+                                                   const double* n, double mu, double* S)
+{
+  const double irL = rcp(UL[0]);
+  const double jr  = UL[0] - Ub[0];
+  double a[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    a[i] = ((UL[1 + i] - Ub[1 + i]) - jr * (UL[1 + i] * irL)) * irL;
+  const double lt = -(2.0 / 3.0) * mu * (a[0] * n[0] + a[1] * n[1] + a[2] * n[2]);
+  S[0] = 2.0 * mu * a[0] * n[0] + lt;
+  S[1] = 2.0 * mu * a[1] * n[1] + lt;
+  S[2] = 2.0 * mu * a[2] * n[2] + lt;
+  S[3] = mu * (a[0] * n[1] + a[1] * n[0]);
+  S[4] = mu * (a[0] * n[2] + a[2] * n[0]);
+  S[5] = mu * (a[1] * n[2] + a[2] * n[1]);
+}
+
 // Volume flux H(Vq, gr) (dgx_physics.hpp): gx[0..4] = d/dVq, gx[5 + 4c + r]
 // = d/dgr[c][r] of sum psiH . H, in reverse mode.  Forward: u = m/rho,
 // dv[c][i] = (gr[c][1+i] - gr[c][0] u_i)/rho, Lv[i][d] = sum_c dv[c][i] Ji[c][d],

@@ -41,7 +41,8 @@ BTYPES = {"periodic": PERIODIC, "freestream": FREESTREAM, "inviscid_wall": INVIS
 RK4, TVDRK3 = 0, 1
 SCHEMES = {"RK4": RK4, "tvdRK3": TVDRK3}
 PREC_FP64, PREC_MIXED = 0, 1
-PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
+PREC_MIXED_BF16 = 2
+PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "mixed_bf16": PREC_MIXED_BF16}
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "HIP error",
           4: "NaN residual norm", 5: "I/O error", 6: "no HIP device"}
@@ -509,8 +510,14 @@ class Solver:
     def precision(self, name):
         _chk(lib().dgx_set_precision(self.ctx, PRECISIONS[name]), self.ctx)
 
-    # what DGX_PREC_MIXED computes in FP32 (the rest stays FP64)
-    MIXED_FORM = "FP32 face trace records; geometry, fluxes, Roe, inverse mass, RK in FP64"
+    # what the mixed forms compute below FP64 (the rest stays FP64)
+    MIXED_FORMS = {
+        "mixed": "FP32 face trace records; p=3 tiled kernel: volume gradient and flux "
+                 "projection in FP32 on the matrix cores (v_mfma_f32_4x4x1f32); geometry, "
+                 "Roe, pressure/energy, viscous physics, inverse mass, RK in FP64",
+        "mixed_bf16": "as mixed, the contractions as 3-term split-BF16 products "
+                      "(v_mfma_f32_4x4x4bf16_1k, FP32 accumulation)",
+    }
 
     # rows per tile of the tiled stage kernel at p = 1, 2, 3
     TILE_ROWS = {1: 1, 2: 3, 3: 2}

@@ -12,8 +12,7 @@
     for later backward passes until ``solver.adjoint_release()`` or
     ``solver.close()``.
 
-Both act on periodic meshes (the adjoint kernels cover interior faces
-only).  States are float64 tensors in the simstate layout U[e][r][bf]:
+Both act on meshes with any boundary types.  States are float64 tensors in the simstate layout U[e][r][bf]:
 device tensors are handed to the library by address after the caller's
 stream is drained, host tensors are copied.  The library runs on its own
 stream and every call returns after completion, so results are ready when

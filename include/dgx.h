@@ -260,8 +260,9 @@ dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* ens, int i, int* persi
 const char* dgx_ensemble_last_error(const dgx_ensemble* ens);
 
 /* ---- discrete adjoint (no counterpart in the reference) --------------------
- * Exact transposes of the device's discrete residual and RK4 stages, for
- * periodic meshes (DGX_ERR_UNSUPPORTED otherwise). */
+ * Exact transposes of the device's discrete residual and RK4 stages, on
+ * meshes with any boundary types (walls, inflow, outflow, free stream: the
+ * boundary states are differentiated with the face physics). */
 
 /* J = volume integral of the kinetic energy |m|^2 / (2 rho) of the context's
  * state, with the (p+1)^3 Gauss rule of the mass matrix. */
@@ -312,17 +313,26 @@ dgx_status dgx_adjoint_release(dgx_ctx* ctx);
 
 typedef enum dgx_precision
 {
-  DGX_PREC_FP64  = 0, /* everything in FP64 (default)                      */
-  DGX_PREC_MIXED = 1  /* FP32 face trace records, the rest in FP64         */
+  DGX_PREC_FP64       = 0, /* everything in FP64 (default)                  */
+  DGX_PREC_MIXED      = 1, /* FP32 trace records; p = 3 tiled kernel: volume
+                              contractions in FP32 on the matrix cores       */
+  DGX_PREC_MIXED_BF16 = 2  /* as MIXED, the contractions as split-BF16
+                              (3-term) products on the BF16 matrix cores    */
 } dgx_precision;
 
-/* DGX_PREC_MIXED stores the face trace records that cross element faces
+/* The mixed forms store the face trace records that cross element faces
  * between stages (the interpolated trace state and the gradient traction,
- * computed in FP64 and rounded once) in FP32; geometry, the Roe and viscous
- * fluxes, pressure/energy recovery, the inverse mass and the RK accumulation
- * stay FP64.  Changing the form drops the stored records (the next stage
- * recomputes them).  The adjoint entry points and ensemble steps need
- * DGX_PREC_FP64 (DGX_ERR_UNSUPPORTED otherwise). */
+ * computed in FP64 and rounded once) in FP32, and in the tiled p = 3 kernel
+ * run the volume contractions (the reference-space gradient of density and
+ * momentum, and the projection of the contravariant volume flux onto the
+ * nodes) on the matrix cores with FP32 accumulation: FP32 operands
+ * (DGX_PREC_MIXED) or each FP32 operand split into three BF16 terms, the six
+ * leading products accumulated (DGX_PREC_MIXED_BF16).  Geometry, the Roe
+ * flux and its eigen-structure, pressure/energy recovery, the viscous flux
+ * physics, the inverse mass and the RK accumulation stay FP64.  Changing the
+ * form drops the stored records (the next stage recomputes them).  The
+ * adjoint entry points and ensemble steps need DGX_PREC_FP64
+ * (DGX_ERR_UNSUPPORTED otherwise). */
 dgx_status dgx_set_precision(dgx_ctx* ctx, int precision);
 dgx_status dgx_get_precision(const dgx_ctx* ctx, int* precision);
 

@@ -5,7 +5,9 @@
   * time stepping: the gradient of the final kinetic energy after N RK4
     steps with respect to the initial state (random directions) and to mu,
     from dgx_adjoint_rk4 with checkpointing, equals central differences of
-    dgx_step + dgx_kinetic_energy.
+    dgx_step + dgx_kinetic_energy;
+  * both on periodic meshes and on meshes with every boundary type (no-slip
+    and inviscid walls, subsonic inflow and outflow, free stream).
 Central differences carry O(eps^2) truncation and O(1e-16 |f| / eps)
 rounding; the tolerances below (1e-6 relative) sit well above both for the
 chosen eps and well below any missing or mis-signed term."""
@@ -110,16 +112,96 @@ def test_adjoint_schedules_agree_bitwise(built, name):
         assert np.array_equal(g, out[0][1]) and np.array_equal(U, out[0][3])
 
 
-def test_adjoint_needs_periodic_mesh(built):
-    c = cases.channel_case(3, 3, 2, p=2, seed=16)
+def perturbed_uniform(mesh, prm, amp, seed):
+    rng = np.random.default_rng(seed)
+    U = np.zeros((mesh.nelem, 5, mesh.nbf))
+    for r in range(5):
+        U[:, r, :] = prm.Ufr[r]
+    U[:, 0, :] *= 1.0 + amp * rng.uniform(-1, 1, (mesh.nelem, mesh.nbf))
+    for r in (1, 2, 3):
+        U[:, r, :] += amp * U[:, 0, :] * prm.c * rng.uniform(-1, 1, (mesh.nelem, mesh.nbf))
+    U[:, 4, :] *= 1.0 + amp * rng.uniform(-1, 1, (mesh.nelem, mesh.nbf))
+    return U.reshape(-1)
+
+
+def bump(p, q, bounds, seed, dt):
+    """Small bump channel (the reference's default geometry, input.cpp:23-66)
+    with a perturbed uniform state at Mach 0.1."""
+    nodes = dgx.bump_nodes(q, 4, 3, 2, 2.0, 1.0, 1.0, 0.0, 0.1, 1.0, 0.05)
+    mesh = dgx.Mesh(p, q, 4, 3, 2, bounds, nodes)
+    prm = dgx.make_reference_state(gamma=1.4, R=287.0, mu=2e-3, c=1.0, rho=1.0, u=0.1, v=0.0,
+                                   w=0.0, eta=6.0)
+    U0 = perturbed_uniform(mesh, prm, 0.02, seed)
+    return cases.Case("bump_p%d_q%d" % (p, q), mesh, prm, U0, "RK4", dt, 4)
+
+
+def boundary_cases():
+    """Every boundary type of boundaries.cpp:26-35 on at least one side."""
+    return {
+        # no-slip walls at y = +-1, periodic in x and z
+        "channel_noslip_p2": cases.channel_case(3, 3, 2, p=2, seed=16, M=0.3, Re=400.0),
+        # inviscid walls, curved q = 2 elements
+        "channel_inviscid_p3q2": cases.channel_case(2, 3, 2, p=3, q=2, seed=17, M=0.3, Re=400.0,
+                                                    wall=dgx.INVISCID_WALL),
+        # subsonic inflow / outflow, inviscid walls over the bump
+        "bump_inout_p2q2": bump(2, 2, [dgx.SUBSONIC_INFLOW, dgx.SUBSONIC_OUTFLOW,
+                                       dgx.INVISCID_WALL, dgx.INVISCID_WALL, dgx.PERIODIC,
+                                       dgx.PERIODIC], 71, 2e-3),
+        # free stream, no-slip wall, inviscid wall and outflow on one mesh
+        "bump_mixed_p1q1": bump(1, 1, [dgx.FREESTREAM, dgx.FREESTREAM, dgx.NO_SLIP_WALL,
+                                       dgx.FREESTREAM, dgx.INVISCID_WALL, dgx.SUBSONIC_OUTFLOW],
+                                72, 4e-3),
+    }
+
+
+@pytest.mark.parametrize("name", ["channel_noslip_p2", "channel_inviscid_p3q2", "bump_inout_p2q2",
+                                  "bump_mixed_p1q1"])
+def test_boundary_residual_vjp_dot_product(built, name):
+    """Residual adjoint on meshes with domain boundaries: the boundary states
+    (and their dependence on the interior trace) are transposed with the
+    face physics."""
+    c = boundary_cases()[name]
+    U = c.U0
+    v = rng_dir(U, 3)
+    w = np.random.default_rng(4).uniform(-1.0, 1.0, U.shape)
+    eps = 1e-6
+    with dgx.Solver(c.mesh, c.params) as s:
+        s.set_state(U + eps * v)
+        _, fp = s.residual()
+        s.set_state(U - eps * v)
+        _, fm = s.residual()
+        s.set_state(U)
+        g = s.residual_vjp(w)
+    lhs = float(w @ ((fp - fm) / (2 * eps)))
+    rhs = float(v @ g)
+    err = abs(lhs - rhs) / abs(rhs)
+    print("  %s: w.Jv (FD) %.12e  v.J^T w %.12e  rel %.2e" % (name, lhs, rhs, err), flush=True)
+    assert err < FD_TOL
+
+
+@pytest.mark.parametrize("name,nsteps,interval", [("channel_noslip_p2", 4, 2),
+                                                  ("bump_inout_p2q2", 3, 0),
+                                                  ("bump_mixed_p1q1", 4, 1)])
+def test_boundary_adjoint_rk4_gradient_matches_fd(built, name, nsteps, interval):
+    c = boundary_cases()[name]
     with dgx.Solver(c.mesh, c.params) as s:
         s.set_state(c.U0)
-        with pytest.raises(dgx.DgxError) as ei:
-            s.adjoint_rk4(c.dt, 2)
-        assert ei.value.status == dgx.ERR_UNSUPPORTED
-        with pytest.raises(dgx.DgxError) as ei:
-            s.residual_vjp(np.ones(c.mesh.state_size))
-        assert ei.value.status == dgx.ERR_UNSUPPORTED
+        J, g, dmu = s.adjoint_rk4(c.dt, nsteps, interval)
+        assert abs(J - final_energy(c, c.U0, nsteps)) <= 1e-14 * abs(J)
+    eps = 1e-6
+    for k in range(2):
+        d = rng_dir(c.U0, 20 + k)
+        fd = (final_energy(c, c.U0 + eps * d, nsteps) - final_energy(c, c.U0 - eps * d, nsteps)) / (2 * eps)
+        ad = float(g @ d)
+        err = abs(fd - ad) / abs(ad)
+        print("  %s dir %d: FD %.12e  adjoint %.12e  rel %.2e" % (name, k, fd, ad, err), flush=True)
+        assert err < FD_TOL
+    mu = c.params.mu
+    dm = 1e-4 * mu
+    fd = (final_energy(c, c.U0, nsteps, mu + dm) - final_energy(c, c.U0, nsteps, mu - dm)) / (2 * dm)
+    err = abs(fd - dmu) / abs(dmu)
+    print("  %s dJ/dmu: FD %.12e  adjoint %.12e  rel %.2e" % (name, fd, dmu, err), flush=True)
+    assert err < FD_TOL
 
 
 def test_stored_stages_kept_between_calls(built):

@@ -1,5 +1,7 @@
-"""The mixed-precision form (DGX_PREC_MIXED: FP32 face trace records, the
-rest FP64) against the FP64 oracle, and its switches.
+"""The mixed-precision forms against the FP64 oracle, and their switches:
+DGX_PREC_MIXED (FP32 face trace records; in the tiled p = 3 kernel the
+volume gradient and flux projection in FP32 on the matrix cores) and
+DGX_PREC_MIXED_BF16 (the same contractions as 3-term split-BF16 products).
 
 Rounding the trace records to FP32 perturbs the face terms by about 6e-8 of
 the trace values.  The residual error scales with the flux terms, not with
@@ -41,6 +43,8 @@ def bump(p, q):
 CASES = {
     "tgv_curved_p3": lambda: cases.tgv_case(n=3, p=3, seed=71, M=0.2, Re=800.0, amp=0.1,
                                             dt=1e-3, nsteps=6),
+    # affine p = 3 (the 64^3 workload's kernels) at Mach 0.1
+    "tgv4_p3": lambda: cases.tgv_case(n=4, p=3, seed=75, dt=4e-3, nsteps=10),
     "tgv4_p2": lambda: cases.tgv_case(n=4, p=2, seed=72, dt=2e-3, nsteps=6),
     "channel_p3": lambda: cases.channel_case(4, 3, 3, p=3, seed=73, nsteps=20),
     "channel_q2_p2": lambda: cases.channel_case(3, 3, 2, p=2, q=2, seed=74, nsteps=20),
@@ -59,8 +63,9 @@ def geometry(orc, m):
     return orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
 
 
+@pytest.mark.parametrize("form", ["mixed", "mixed_bf16"])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_mixed_residual_and_solution(name, orc, built):
+def test_mixed_residual_and_solution(name, form, orc, built):
     c = CASES[name]()
     m = c.mesh
     print("case", name, "nelem", m.nelem, "p", m.p, "q", m.q, c.scheme, "steps", c.nsteps,
@@ -74,16 +79,16 @@ def test_mixed_residual_and_solution(name, orc, built):
         _, f_r = s.residual()
         s.set_state(c.U0)
         _, f64 = s.residual()
-        s.precision = "mixed"
-        assert s.precision == "mixed"
+        s.precision = form
+        assert s.precision == form
         _, fmx = s.residual()
         n_g = s.step(c.dt, c.scheme, c.nsteps)
         U_g = s.get_state()
     ef, eU = rel(fmx, f_o), rel(U_g, U_o)
     en = float(np.max(np.abs(n_g - n_o) / np.abs(n_o)))
     cond = rel(f_r, f64)
-    print("  mixed: rel err f %.3e  U %.3e  norms %.3e  (FP64 f %.3e; FP64 f of the state "
-          "rounded to FP32 %.3e)" % (ef, eU, en, rel(f64, f_o), cond), flush=True)
+    print("  %s: rel err f %.3e  U %.3e  norms %.3e  (FP64 f %.3e; FP64 f of the state "
+          "rounded to FP32 %.3e)" % (form, ef, eU, en, rel(f64, f_o), cond), flush=True)
     assert ef < KAPPA_FACTOR * cond and en < KAPPA_FACTOR * cond
     assert eU < SOL_TOL
     assert rel(fmx, f64) > 1e-12
@@ -143,3 +148,25 @@ def test_mixed_form_refuses_adjoint_and_ensembles(built):
         with pytest.raises(dgx.DgxError) as ei:
             en.step("RK4", [c.dt, c.dt], 1)
         assert ei.value.status == dgx.ERR_UNSUPPORTED
+
+
+def test_matrix_core_forms_differ_at_p3(built):
+    """At p = 3 the two mixed forms run different contraction arithmetic
+    (FP32 vs split-BF16 matrix-core products): their residuals differ from
+    each other and from the FP32-records-only one-row kernel, within the
+    mixed bar of one another."""
+    c = CASES["tgv4_p3"]()
+    with dgx.Solver(c.mesh, c.params) as s:
+        s.set_state(c.U0)
+        out = {}
+        for form in ("mixed", "mixed_bf16"):
+            s.precision = form
+            _, out[form] = s.residual()
+        s.tile = 0
+        s.precision = "mixed"
+        _, rec_only = s.residual()
+    d1 = rel(out["mixed"], out["mixed_bf16"])
+    d2 = rel(out["mixed"], rec_only)
+    print("  mixed vs mixed_bf16 %.3e, mixed vs FP32 records only %.3e" % (d1, d2), flush=True)
+    assert d1 > 0.0 and d2 > 0.0
+    assert d1 < 1e-5 and d2 < 1e-5
