@@ -13,9 +13,10 @@ Prints one JSON line:
   roofline   = algorithmic HBM bytes per stage-kernel launch / average launch
                time (HIP events on the library's stream, steps without the
                norm) vs 8 TB/s
-  mixed, mixed_bf16 = the same workload in the two mixed-precision forms
-               (FP32 trace records; volume contractions on the matrix cores
-               in FP32, or in 3-term split BF16): throughput, speedup over
+  mixed, mixed_bf16, mixed_records = the same workload in the three
+               mixed-precision forms (FP32 trace records; volume
+               contractions on the matrix cores in FP32, or in 3-term split
+               BF16, or in FP64): throughput, speedup over
                FP64, and after the workload's 100 RK4 steps from the initial
                state per-field L2/Linf errors and the kinetic-energy
                dissipation rate against the FP64 run
@@ -121,15 +122,16 @@ def cpu_baseline(seconds_hint=20.0, threads=None):
 
 def run_ensemble(a, h):
     """64 independent periodic cubes (sizes, p, Mach, Re from seed 3) advanced
-    together.  Default mode: per RK stage ONE launch of the persistent stage
-    kernel for all 64 members (p = 1..3 in one kernel, workgroups taking
-    work units from an atomic counter), one RK step of the whole ensemble
-    replayed from a captured hipGraph, occupancy target and tile (elements
-    per workgroup, nodes per lane) from the autotuner.  Also timed: the
-    batched mode (one launch per (p, q) group and stage, autotuned), and the
-    same cases stepped one after the other with the single-case stepper
-    ("64 sequential plain launches").  The step count is the workload's
-    (50); --steps overrides it."""
+    together, one RK step of the whole ensemble replayed from a captured
+    hipGraph.  The autotuner picks, per launch group, the occupancy target
+    and tile (elements per workgroup, nodes per lane), then the layout: ONE
+    launch per RK stage of the ensemble stage kernel for all 64 members
+    (p = 1..3 in one kernel, one workgroup per work unit, largest members
+    first), one such launch per order, or one batched launch per (p, q)
+    group.  Every layout is timed; `value` is the chosen one's.  Also
+    timed: the same cases stepped one after the other with the single-case
+    stepper ("64 sequential plain launches").  The step count is the
+    workload's (50); --steps overrides it."""
     cs = cases.config3()
     steps = a.steps if a.steps_given else cs[0].nsteps
     dts = [c.dt for c in cs]
@@ -141,12 +143,15 @@ def run_ensemble(a, h):
     res = {}
     with dgx.Ensemble([c.mesh for c in cs], [c.params for c in cs]) as en:
         stream = C.c_void_p(en.stream())
-        for mode in (False, True):  # the default (persistent) last, its states are checked
-            en.set_persistent(mode)
+        for m, c in zip(en.members, cs):
+            m.set_state(c.U0)
+        waves = en.autotune("RK4", dts, reps=2) if a.autotune else None
+        chosen = en.member_tuning(0)[0]
+        # the chosen layout last: its states are checked
+        for mode in [x for x in (0, 2, 1) if x != chosen] + [chosen]:
+            en.set_layout(mode)
             for m, c in zip(en.members, cs):
                 m.set_state(c.U0)
-            waves = en.autotune("RK4", dts, reps=2) if a.autotune else None
-            tuning = sorted(set(en.member_tuning(i) for i in range(len(cs))))
             en.step("RK4", dts, max(a.warmup, 1), graph=True)
             for m, c in zip(en.members, cs):
                 m.set_state(c.U0)
@@ -158,7 +163,8 @@ def run_ensemble(a, h):
             en.sync()
             ms = C.c_float()
             h.hipEventElapsedTime(C.byref(ms), ev0, ev1)
-            res[mode] = (ms.value * 1e-3, waves, tuning)
+            tuning = sorted(set(en.member_tuning(i) for i in range(len(cs))))
+            res[mode] = (ms.value * 1e-3, tuning)
         finite = all(bool(dgx.np.isfinite(m.get_state()).all()) for m in en.members)
         # the same cases one after the other, single-case launches
         for m, c in zip(en.members, cs):
@@ -172,9 +178,11 @@ def run_ensemble(a, h):
             m.sync()
         h.hipDeviceSynchronize()
         t_seq = time.perf_counter() - t0
-    t_ens, waves, tuning = res[True]
-    t_bat = res[False][0]
+    t_ens, tuning = res[chosen]
     work = dof * 4 * steps
+    layout_names = {1: "one launch per stage for all orders",
+                    2: "one launch per order and stage",
+                    0: "one batched launch per (p, q) group and stage"}
     value = work / t_ens / 1e9
     bpd = 170.0  # RK4 with accumulator, affine meshes (no geometry bytes)
     out = {
@@ -198,15 +206,16 @@ def run_ensemble(a, h):
             "cases": len(cs), "dof": dof, "scheme": "RK4",
             "groups": sorted(set((c.mesh.p, c.mesh.q) for c in cs)),
             "stage_waves_per_member": waves,
-            "launch_tuning": [{"persistent": p_, "tile": t_, "waves": w_} for p_, t_, w_ in tuning],
-            "parallelism": "single GPU, one persistent launch per stage + hipGraph",
+            "launch_tuning": [{"layout": p_, "tile": t_, "waves": w_} for p_, t_, w_ in tuning],
+            "layout": layout_names[chosen] + " (autotuned), hipGraph",
+            "parallelism": "single GPU",
         },
-        "batched_per_group": {
-            "value": dof * 4 * steps / t_bat / 1e9,
-            "unit": "GDOF-stage/s",
-            "ms_per_step": 1e3 * t_bat / steps,
-            "what": "one batched launch per (p, q) group and stage, autotuned, hipGraph",
-        },
+        "layouts": {
+            layout_names[m]: {"value": work / res[m][0] / 1e9, "unit": "GDOF-stage/s",
+                              "ms_per_step": 1e3 * res[m][0] / steps,
+                              "launch_tuning": [{"layout": p_, "tile": t_, "waves": w_}
+                                                for p_, t_, w_ in res[m][1]]}
+            for m in res},
         "sequential": {
             "value": work / t_seq / 1e9,
             "unit": "GDOF-stage/s",
@@ -216,7 +225,8 @@ def run_ensemble(a, h):
         "speedup_vs_sequential": t_seq / t_ens,
         "roofline": {
             "bound": "hbm",
-            "kernel": "dgx_ens_stage_kernel<W,TILE,AFF> (all members, one launch per stage)",
+            "kernel": ("dgx_rhs_batch_kernel<P,Q,W,AFF>" if chosen == 0 else
+                       "dgx_ens_stage_kernel<W,TILE,AFF,PS>") + " (whole ensemble step)",
             "achieved": bpd * work / t_ens / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -230,11 +240,12 @@ def run_ensemble(a, h):
 
 def run_adjoint(a, h):
     """Discrete adjoint on the 32^3 p=2 periodic random-smooth-field case:
-    forward time of N RK4 steps, time of dgx_adjoint_rk4 (default schedule:
-    every stage state and its trace records kept from the forward sweep when
-    they fit in HBM, then the reverse sweep; gradients with respect to the
-    initial state and to mu), their ratio, the checkpointed schedules
-    (ceil(sqrt(N)) and every step) for comparison, and
+    forward time of N RK4 steps, time of dgx_adjoint_rk4 with the
+    workload's schedule (checkpoints every ceil(sqrt(N)) steps; gradients
+    with respect to the initial state and to mu), their ratio, the default
+    schedule (every stage state and its trace records kept from the forward
+    sweep when they fit in HBM) and checkpoints at every step for
+    comparison, and
     the checks: central differences (eps = 1e-6) of the final kinetic energy
     along 20 seeded random directions, central differences in mu, and a
     residual-level dot-product test of the tangent-linear residual against
@@ -262,8 +273,13 @@ def run_adjoint(a, h):
         t0 = time.perf_counter()
         _, g0, _ = s.adjoint_rk4(c.dt, N, want_mu=False)
         t_adj_state = time.perf_counter() - t0
-        # checkpointed schedules: every ceil(sqrt(N)) steps, every step
+        # checkpointed schedules: every ceil(sqrt(N)) steps (the workload's
+        # schedule, with and without dJ/dmu), every step
         K = int(dgx.np.ceil(dgx.np.sqrt(N)))
+        s.set_state(U0)
+        t0 = time.perf_counter()
+        Jk, gkm, dmuk = s.adjoint_rk4(c.dt, N, interval=K)
+        t_adj_sqrt = time.perf_counter() - t0
         s.set_state(U0)
         t0 = time.perf_counter()
         _, gk, _ = s.adjoint_rk4(c.dt, N, interval=K, want_mu=False)
@@ -273,7 +289,8 @@ def run_adjoint(a, h):
         _, g1, _ = s.adjoint_rk4(c.dt, N, interval=1, want_mu=False)
         t_adj_all = time.perf_counter() - t0
         same = float(max(dgx.np.max(dgx.np.abs(g1 - g)), dgx.np.max(dgx.np.abs(gk - g)),
-                         dgx.np.max(dgx.np.abs(g0 - g))))
+                         dgx.np.max(dgx.np.abs(g0 - g)), dgx.np.max(dgx.np.abs(gkm - g)),
+                         abs(dmuk - dmu), abs(Jk - J)))
         # the default schedule keeps every stage when this fits in 3/4 of free HBM
         free, total = C.c_size_t(), C.c_size_t()
         h.hipMemGetInfo(C.byref(free), C.byref(total))
@@ -337,13 +354,18 @@ def run_adjoint(a, h):
             "objective": "final kinetic energy (Gauss rule of the mass matrix)",
             "J": J, "J_forward_run": J_fwd,
             "forward_s": t_fwd,
-            "adjoint_s": t_adj,
-            "adjoint_over_forward": t_adj / t_fwd,
-            "schedule": "stored stages" if stored else "checkpoints every ceil(sqrt(N))",
-            "stored_stage_bytes": need,
-            "adjoint_state_only_s": t_adj_state,
-            "adjoint_state_only_over_forward": t_adj_state / t_fwd,
+            "adjoint_s": t_adj_sqrt,
+            "adjoint_over_forward": t_adj_sqrt / t_fwd,
+            "schedule": "checkpoints every ceil(sqrt(N)) steps; per segment one replay into "
+                        "a store of its stage states, then the reverse sweep (state and mu "
+                        "gradients)",
             "checkpointed_sqrtN_state_only_over_forward": t_adj_chk / t_fwd,
+            "default_schedule": "stored stages" if stored else "checkpoints every ceil(sqrt(N))",
+            "default_s": t_adj,
+            "default_over_forward": t_adj / t_fwd,
+            "stored_stage_bytes": need,
+            "default_state_only_s": t_adj_state,
+            "default_state_only_over_forward": t_adj_state / t_fwd,
             "checkpointed_every_step_state_only_over_forward": t_adj_all / t_fwd,
             "schedules_max_abs_grad_difference": same,
             "checkpoint_interval": K,
@@ -543,7 +565,7 @@ def main():
     mixed = {}
     if not a.no_mixed:
         ref = fp64_reference(s, case)
-        for form in ("mixed", "mixed_bf16"):
+        for form in ("mixed", "mixed_bf16", "mixed_records"):
             mixed[form] = run_mixed(a, h, s, case, value, stream, ev0, ev1, norms_dev, form, ref)
     h.hipFree(norms_dev)
     s.close()

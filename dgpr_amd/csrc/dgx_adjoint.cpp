@@ -293,53 +293,59 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   // Schedule.  interval == 0: keep every stage state with its trace records
   // (and f for dJ/dmu) from the forward sweep when that fits in three
   // quarters of the free device memory, so the reverse sweep recomputes
-  // nothing; otherwise (or for interval >= 1) checkpoint every K steps.
-  // The store stays with the context for later calls (an optimisation loop
-  // calls this repeatedly; freeing tens of GB has taken seconds) until
-  // dgx_adjoint_release or dgx_destroy, and is replaced only when too small.
+  // nothing; otherwise (or for interval >= 1) checkpoint every K steps, and
+  // per segment replay its K steps once into a store of K steps' stage
+  // states, which the reverse sweep then reads as it reads the full store
+  // (when even that does not fit, rebuild each step's stages from the
+  // segment's step-start states).  The store stays with the context for
+  // later calls (an optimisation loop calls this repeatedly; freeing tens of
+  // GB has taken seconds) until dgx_adjoint_release or dgx_destroy, and is
+  // replaced only when too small.
   const size_t per_step = 4 * (n + ntr) + (dJdmu ? 4 * n : 0);
-  double* store = nullptr;
-  if (interval == 0)
-  {
-    const size_t need = (size_t)nsteps * per_step * sizeof(double);
-    if (c->adj_store && c->adj_store_bytes >= need)
-      store = c->adj_store;
-    else
-    {
-      size_t fr = 0, tot = 0;
-      if (hipMemGetInfo(&fr, &tot) == hipSuccess &&
-          (double)need <= 0.75 * ((double)fr + (double)c->adj_store_bytes))
-      {
-        AHIP(c, hipStreamSynchronize(c->stream));
-        if (c->adj_store)
-          AHIP(c, hipFree(c->adj_store));
-        c->adj_store = nullptr;
-        c->adj_store_bytes = 0;
-        double* p = nullptr;
-        if (hipMalloc(&p, need) == hipSuccess)
-        {
-          c->adj_store = store = p;
-          c->adj_store_bytes = need;
-        }
-        else
-          (void)hipGetLastError();  // fall back to checkpointing
-      }
-    }
-  }
-  const bool store_all = store != nullptr;
-  if (!store_all && c->adj_store)
-  {
-    // this call checkpoints: a held store it does not use would only crowd
-    // the checkpoint buffers out of memory
-    AHIP(c, hipStreamSynchronize(c->stream));
-    AHIP(c, hipFree(c->adj_store));
-    c->adj_store = nullptr;
-    c->adj_store_bytes = 0;
-  }
   const uint64_t K = interval ? interval : (uint64_t)std::ceil(std::sqrt((double)nsteps));
+  auto acquire_store = [&](uint64_t steps) -> double* {
+    const size_t need = (size_t)steps * per_step * sizeof(double);
+    if (c->adj_store && c->adj_store_bytes >= need)
+      return c->adj_store;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess ||
+        (double)need > 0.75 * ((double)fr + (double)c->adj_store_bytes))
+      return nullptr;
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+      return nullptr;
+    if (c->adj_store)
+      (void)hipFree(c->adj_store);
+    c->adj_store       = nullptr;
+    c->adj_store_bytes = 0;
+    double* p          = nullptr;
+    if (hipMalloc(&p, need) != hipSuccess)
+    {
+      (void)hipGetLastError();  // fall back to the next schedule
+      return nullptr;
+    }
+    c->adj_store       = p;
+    c->adj_store_bytes = need;
+    return p;
+  };
+  double* store = (interval == 0) ? acquire_store(nsteps) : nullptr;
+  const bool store_all = store != nullptr;
+  uint64_t slots = store_all ? nsteps : 0;  // steps the store holds
+  if (!store_all)
+  {
+    if (c->adj_store && c->adj_store_bytes < (size_t)K * per_step * sizeof(double))
+    {
+      // too small for a segment: it would only crowd the checkpoint
+      // buffers out of memory
+      AHIP(c, hipStreamSynchronize(c->stream));
+      AHIP(c, hipFree(c->adj_store));
+      c->adj_store       = nullptr;
+      c->adj_store_bytes = 0;
+    }
+    // checkpoints are allocated first, the segment store from what is left
+  }
   auto Sp = [&](uint64_t s, int i) { return store + (s * 4 + i) * (n + ntr); };
   auto Tp = [&](uint64_t s, int i) { return Sp(s, i) + n; };
-  auto Kp = [&](uint64_t s, int i) { return store + nsteps * 4 * (n + ntr) + (s * 4 + i) * n; };
+  auto Kp = [&](uint64_t s, int i) { return store + slots * 4 * (n + ntr) + (s * 4 + i) * n; };
 
   // reverse of one step: stage states Si with records Ti and (dJ/dmu only)
   // their right-hand sides Ki = f(S_i); lam, the adjoint of the step's end
@@ -380,24 +386,30 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   std::vector<DevBuf> chk(nchk);
   for (auto& b : chk)
     AHIP(c, alloc(b, n));
-
-  // forward sweep
-  if (store_all)
+  bool seg_store = false;
+  if (!store_all)
   {
-    // every stage state and its records written into the store: the stage
-    // arguments of stage_plan (RK4) with other output buffers
-    AHIP(c, copy_state(c, Sp(0, 0), c->U));
+    store     = acquire_store(K);
+    seg_store = store != nullptr;
+    slots     = seg_store ? K : 0;
+  }
+
+  // ns steps from the state V0, every stage state and its records (and f
+  // for dJ/dmu) written into store slots 0..ns-1, the end state to Vend:
+  // the stage arguments of stage_plan (RK4) with other output buffers
+  auto forward_store = [&](const double* V0, uint64_t ns, double* Vend) -> dgx_status {
+    AHIP(c, copy_state(c, Sp(0, 0), V0));
     AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), c->phys, Sp(0, 0), Tp(0, 0), c->status,
                            c->stream));
     const int accmode[4] = {1, 2, 2, 3};
-    for (uint64_t s = 0; s < nsteps; ++s)
+    for (uint64_t s = 0; s < ns; ++s)
       for (int i = 0; i < 4; ++i)
       {
         StageArgs sa;
-        sa.Vin = Sp(s, i);
-        sa.Tin = Tp(s, i);
+        sa.Vin   = Sp(s, i);
+        sa.Tin   = Tp(s, i);
         sa.Vbase = Sp(s, 0);
-        sa.acc = c->acc;
+        sa.acc   = c->acc;
         if (i < 3)
         {
           sa.Vout = Sp(s, i + 1);
@@ -405,19 +417,42 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
         }
         else
         {
-          sa.Vout = (s + 1 < nsteps) ? Sp(s + 1, 0) : final_state.p;
-          sa.Tout = (s + 1 < nsteps) ? Tp(s + 1, 0) : nullptr;
+          sa.Vout = (s + 1 < ns) ? Sp(s + 1, 0) : Vend;
+          sa.Tout = (s + 1 < ns) ? Tp(s + 1, 0) : nullptr;
         }
-        sa.fout = dJdmu ? Kp(s, i) : nullptr;
-        sa.ca = 1.0;
-        sa.cb = 0.0;
-        sa.cf = cstage[i];
+        sa.fout    = dJdmu ? Kp(s, i) : nullptr;
+        sa.ca      = 1.0;
+        sa.cb      = 0.0;
+        sa.cf      = cstage[i];
         sa.accmode = accmode[i];
         sa.trace32 = 0;
         sa.ctype   = 0;
         AHIP(c, rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa, c->lam, c->status,
                              c->stage_waves, c->stream, c->stage_tile));
       }
+    return DGX_OK;
+  };
+  // reverse of store slots ns-1..0
+  auto reverse_store = [&](uint64_t ns) -> dgx_status {
+    for (uint64_t s = ns; s-- > 0;)
+    {
+      double* const Si[4] = {Sp(s, 0), Sp(s, 1), Sp(s, 2), Sp(s, 3)};
+      double* const Ti[4] = {Tp(s, 0), Tp(s, 1), Tp(s, 2), Tp(s, 3)};
+      double* const Ki[4] = {dJdmu ? Kp(s, 0) : nullptr, dJdmu ? Kp(s, 1) : nullptr,
+                             dJdmu ? Kp(s, 2) : nullptr, dJdmu ? Kp(s, 3) : nullptr};
+      dgx_status st = reverse_step(Si, Ti, Ki);
+      if (st != DGX_OK)
+        return st;
+    }
+    return DGX_OK;
+  };
+
+  // forward sweep
+  if (store_all)
+  {
+    dgx_status st = forward_store(c->U, nsteps, final_state.p);
+    if (st != DGX_OK)
+      return st;
     AHIP(c, copy_state(c, c->U, final_state.p));
     c->tu_valid = false;
   }
@@ -442,13 +477,20 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   // reverse sweep
   if (store_all)
   {
-    for (uint64_t s = nsteps; s-- > 0;)
+    dgx_status st = reverse_store(nsteps);
+    if (st != DGX_OK)
+      return st;
+  }
+  else if (seg_store)
+  {
+    // per segment: replay its steps once into the store, reverse them
+    for (uint64_t sg = nchk; sg-- > 0;)
     {
-      double* const Si[4] = {Sp(s, 0), Sp(s, 1), Sp(s, 2), Sp(s, 3)};
-      double* const Ti[4] = {Tp(s, 0), Tp(s, 1), Tp(s, 2), Tp(s, 3)};
-      double* const Ki[4] = {dJdmu ? Kp(s, 0) : nullptr, dJdmu ? Kp(s, 1) : nullptr,
-                             dJdmu ? Kp(s, 2) : nullptr, dJdmu ? Kp(s, 3) : nullptr};
-      dgx_status st = reverse_step(Si, Ti, Ki);
+      const uint64_t s0 = sg * K, s1 = (s0 + K < nsteps) ? s0 + K : nsteps;
+      // the segment's end state is not needed again: acur is free here
+      dgx_status st = forward_store(chk[sg].p, s1 - s0, acur.p);
+      if (st == DGX_OK)
+        st = reverse_store(s1 - s0);
       if (st != DGX_OK)
         return st;
     }

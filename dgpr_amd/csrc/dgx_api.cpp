@@ -257,7 +257,8 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
       return bail("hipMalloc(state)", e);
   if ((e = hipMalloc(&c->nodes, nn * sizeof(double))) != hipSuccess)
     return bail("hipMalloc(nodes)", e);
-  if ((e = hipMalloc(&c->partial, (size_t)c->npartial * sizeof(double))) != hipSuccess)
+  if ((e = hipMalloc(&c->partial, ((size_t)c->npartial + sum_scratch(c->npartial)) *
+                                       sizeof(double))) != hipSuccess)
     return bail("hipMalloc(partial)", e);
   if ((e = hipMalloc(&c->sum, sizeof(double))) != hipSuccess)
     return bail("hipMalloc(sum)", e);
@@ -278,7 +279,8 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
   c->stage_tile  = (c->p == 3 && c->q == 1) ? 1 : 0;
   c->stage_waves = c->stage_tile ? (c->affine ? 3 : 2) : default_stage_waves(c->p, c->q);
   c->prec = DGX_PREC_FP64;
-  c->bytes = 5 * sb + 3 * tb + (nn + nlam + (size_t)c->npartial) * sizeof(double) + 16;
+  c->bytes = 5 * sb + 3 * tb +
+             (nn + nlam + (size_t)c->npartial + sum_scratch(c->npartial)) * sizeof(double) + 16;
   if ((e = hipMemcpyAsync(c->nodes, mesh->nodes, nn * sizeof(double), hipMemcpyHostToDevice,
                           c->stream)) != hipSuccess)
     return bail("hipMemcpy(nodes)", e);
@@ -543,12 +545,12 @@ static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm,
     if (k == 0 && want_norm)
     {
       const double* f1 = (scheme == DGX_RK4) ? c->acc : c->fs;
-      if ((e = resid_out_dispatch(c->p, c->q, mesh_args(c), f1, nullptr, nullptr, c->partial,
-                                  c->stream)) != hipSuccess)
+      if ((e = norm_partials_dispatch(c->p, c->q, mesh_args(c), f1, c->partial, c->stream)) !=
+          hipSuccess)
         return e;
-      e = norm_dev ? norm_dispatch(c->partial, c->npartial, 1.0 / (double)c->nstate, norm_dev,
-                                   c->stream)
-                   : sum_dispatch(c->partial, c->npartial, c->sum, c->stream);
+      const int nb = norm_blocks(c->p, c->nelem);
+      e = norm_dev ? norm_dispatch(c->partial, nb, 1.0 / (double)c->nstate, norm_dev, c->stream)
+                   : sum_dispatch(c->partial, nb, c->sum, c->stream);
       if (e != hipSuccess)
         return e;
     }
@@ -612,7 +614,7 @@ dgx_status dgx_step(dgx_ctx* c, int scheme, double dt, uint64_t nsteps, double* 
 dgx_status dgx_set_precision(dgx_ctx* c, int precision)
 {
   if (!c || (precision != DGX_PREC_FP64 && precision != DGX_PREC_MIXED &&
-             precision != DGX_PREC_MIXED_BF16))
+             precision != DGX_PREC_MIXED_BF16 && precision != DGX_PREC_MIXED_RECORDS))
     return DGX_ERR_INVALID;
   if (precision != c->prec)
   {

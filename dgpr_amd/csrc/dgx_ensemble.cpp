@@ -4,19 +4,22 @@
 // keeps one ordinary context per member (mesh, state, stage buffers, trace
 // records, lifting coefficients) and advances all members together.  Per RK
 // stage:
-//  * persistent mode (default): all q = 1 members, of every order, run as
-//    ONE launch of the persistent stage kernel (dgx_ensemble_kernels.hip),
-//    whose workgroups take work units (rows or tiles of one member's
-//    elements) from an atomic counter, largest members first; affine and
-//    curved members form two such launches, q > 1 members one batched launch
-//    per (p, q);
-//  * batched mode: one launch of the batched stage kernel per (p, q, affine)
-//    group (each grid block finds its member in a sorted item table).
+//  * one launch for all orders (layout 1, the default): per affine class,
+//    all q = 1 members of every order run as ONE launch of the ensemble
+//    stage kernel (dgx_ensemble_kernels.hip) over the concatenation of
+//    their work units (rows or tiles of one member's elements), largest
+//    members first; q > 1 members run one batched launch per (p, q);
+//  * one launch per order (layout 2): the same kernel, one launch per
+//    (order, affine) class of the q = 1 members (each launch holds one
+//    order's body only);
+//  * batched (layout 0): one launch of the batched one-row stage kernel per
+//    (p, q, affine) group (each grid block finds its member in a sorted
+//    item table).
 // One RK step of the whole ensemble can be captured in a hipGraph and
-// replayed (kernels only: the unit counters are cleared by a kernel).  The
-// autotuner picks, per launch group, the occupancy target and, for the
-// persistent kernel, the tile (one-row units or tiled units: elements per
-// workgroup and nodes per lane), by timing.
+// replayed (kernels only).  The autotuner picks, per launch group, the
+// occupancy target and, for the ensemble kernel, the tile (one-row units or
+// tiled units: elements per workgroup and nodes per lane), then the layout,
+// by timing.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,14 +38,13 @@ namespace
 {
 struct EnsGroup
 {
-  int p, q;      // p = 0 for a persistent group (q = 1 members of any order)
+  int p, q;      // p = 0 for a one-launch group of q = 1 members of any order
   int affine;    // every member an affine q = 1 mesh
-  bool persistent;
+  bool unit_grid;  // the ensemble stage kernel (layouts 1, 2), else batched
   std::vector<int> members;
-  int nblocks;   // grid blocks (batched) or work units (persistent)
+  int nblocks;   // grid blocks (batched) or work units (ensemble kernel)
   int waves;
-  int tile;      // persistent groups: 0 one-row units, 1 tiled units
-  int grid;      // persistent groups: resident workgroups launched
+  int tile;      // ensemble kernel: 0 one-row units, 1 tiled units
   BatchItem* items[4];  // device item tables, one per stage
 };
 }  // namespace
@@ -60,9 +62,7 @@ struct dgx_ensemble
   int nstages;
   std::vector<double> dt;
   std::vector<Phys> phys;  // each member's physics when the items were built
-  bool persistent;         // run q = 1 members through the persistent kernel
-  int* counters;           // persistent-kernel unit counters [group][stage]
-  int ncu;                 // compute units of the device
+  int layout;              // 0 batched, 1 one launch for all orders, 2 one per order
   bool graph_ready;
   hipGraph_t graph;
   hipGraphExec_t exec;
@@ -99,10 +99,14 @@ static void drop_graph(dgx_ensemble* en)
   }
 }
 
-// Whether group g runs in the current mode.
+// Whether group g runs in the current layout.
 static bool active(const dgx_ensemble* en, const EnsGroup& g)
 {
-  return en->persistent ? (g.persistent || g.q != 1) : !g.persistent;
+  if (en->layout == 0)
+    return !g.unit_grid;
+  if (!g.unit_grid)
+    return g.q != 1;
+  return (en->layout == 1) == (g.p == 0);
 }
 
 // True when some member's physics changed (dgx_set_params on a member
@@ -146,14 +150,9 @@ static dgx_status plan(dgx_ensemble* en, int scheme, const double* dt)
         it.p = c->p;
         host[s].push_back(it);
       }
-      block0 += g.persistent ? ens_units(c->p, c->nelem, g.tile) : stage_blocks(c->p, c->nelem);
+      block0 += g.unit_grid ? ens_units(c->p, c->nelem, g.tile) : stage_blocks(c->p, c->nelem);
     }
     g.nblocks = block0;
-    if (g.persistent)
-    {
-      const int per_cu = ens_blocks_per_cu(g.waves, g.tile, g.affine != 0);
-      g.grid = std::max(1, std::min(block0, en->ncu * std::max(per_cu, 1)));
-    }
     EHIP(en, hipStreamSynchronize(en->stream));  // no launch may still read the tables
     for (int s = 0; s < ns; ++s)
       EHIP(en, hipMemcpy(g.items[s], host[s].data(), sizeof(BatchItem) * host[s].size(),
@@ -178,10 +177,10 @@ static hipError_t enqueue_stage(dgx_ensemble* en, int s, int only_group = -1)
     EnsGroup& g = en->groups[gi];
     if (!active(en, g))
       continue;
-    hipError_t e = g.persistent
+    hipError_t e = g.unit_grid
                        ? ens_stage_dispatch(g.items[s], (int)g.members.size(), g.nblocks,
-                                            en->counters + 4 * gi + s, en->status, g.waves,
-                                            g.tile, g.affine != 0, g.grid, en->stream)
+                                            en->status, g.waves, g.tile, g.affine != 0, g.p,
+                                            en->stream)
                        : rhs_batch_dispatch(g.p, g.q, g.items[s], (int)g.members.size(),
                                             g.nblocks, en->status, g.waves, g.affine != 0,
                                             en->stream);
@@ -191,19 +190,8 @@ static hipError_t enqueue_stage(dgx_ensemble* en, int s, int only_group = -1)
   return hipSuccess;
 }
 
-static hipError_t reset_counters(dgx_ensemble* en)
-{
-  return ens_reset_dispatch(en->counters, 4 * (int)en->groups.size(), en->stream);
-}
-
 static hipError_t enqueue_step(dgx_ensemble* en)
 {
-  if (en->persistent)
-  {
-    hipError_t e = reset_counters(en);
-    if (e != hipSuccess)
-      return e;
-  }
   for (int s = 0; s < en->nstages; ++s)
   {
     hipError_t e = enqueue_stage(en, s);
@@ -237,7 +225,6 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
   en->planned = false;
   en->graph_ready = false;
   en->status = nullptr;
-  en->counters = nullptr;
   en->stream = nullptr;
   en->err[0] = 0;
   for (int i = 0; i < ncase; ++i)
@@ -253,36 +240,27 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
     en->ctx.push_back(c);
   }
   en->device = en->ctx[0]->device;
-  en->persistent = true;
-  en->counters = nullptr;
-  en->ncu = 1;
-  {
-    int ncu = 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, en->device) ==
-            hipSuccess &&
-        ncu > 0)
-      en->ncu = ncu;
-  }
-  auto group_of = [&](bool persistent, int p, int q, int affine) -> EnsGroup& {
+  en->layout = 1;
+  auto group_of = [&](bool unit_grid, int p, int q, int affine) -> EnsGroup& {
     for (EnsGroup& x : en->groups)
-      if (x.persistent == persistent && x.p == p && x.q == q && x.affine == affine)
+      if (x.unit_grid == unit_grid && x.p == p && x.q == q && x.affine == affine)
         return x;
     EnsGroup ng;
     ng.p = p;
     ng.q = q;
     ng.affine = affine;
-    ng.persistent = persistent;
+    ng.unit_grid = unit_grid;
     ng.nblocks = 0;
-    ng.waves = persistent ? 3 : default_stage_waves(p, q);
+    ng.waves = unit_grid ? 3 : default_stage_waves(p, q);
     ng.tile = 0;
-    ng.grid = 1;
     for (int s = 0; s < 4; ++s)
       ng.items[s] = nullptr;
     en->groups.push_back(ng);
     return en->groups.back();
   };
-  // batched groups per (p, q, affine); persistent groups per affine class of
-  // the q = 1 members, largest members (most elements x nodes) first
+  // batched groups per (p, q, affine); one-launch groups per affine class of
+  // the q = 1 members, and per (order, affine class), largest members (most
+  // elements x nodes) first
   for (int i = 0; i < ncase; ++i)
   {
     dgx_ctx* c = en->ctx[i];
@@ -298,7 +276,10 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
   {
     dgx_ctx* c = en->ctx[i];
     if (c->q == 1)
+    {
       group_of(true, 0, 1, c->affine).members.push_back(i);
+      group_of(true, c->p, 1, c->affine).members.push_back(i);
+    }
   }
   auto bail = [&](const char* what, hipError_t e) {
     snprintf(g_ens_err, sizeof(g_ens_err), "%s failed: %s", what, hipGetErrorString(e));
@@ -314,8 +295,6 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
         return bail("hipMalloc(items)", e);
   if ((e = hipMalloc(&en->status, sizeof(int))) != hipSuccess)
     return bail("hipMalloc(status)", e);
-  if ((e = hipMalloc(&en->counters, sizeof(int) * 4 * en->groups.size())) != hipSuccess)
-    return bail("hipMalloc(counters)", e);
   if ((e = hipMemsetAsync(en->status, 0, sizeof(int), en->stream)) != hipSuccess)
     return bail("hipMemset(status)", e);
   if ((e = hipStreamSynchronize(en->stream)) != hipSuccess)
@@ -337,8 +316,6 @@ dgx_status dgx_ensemble_destroy(dgx_ensemble* en)
         (void)hipFree(g.items[s]);
   if (en->status)
     (void)hipFree(en->status);
-  if (en->counters)
-    (void)hipFree(en->counters);
   for (dgx_ctx* c : en->ctx)
     dgx_destroy(c);
   if (en->stream)
@@ -446,58 +423,91 @@ dgx_status dgx_ensemble_autotune(dgx_ensemble* en, int scheme, const double* dt,
   hipEvent_t e0, e1;
   EHIP(en, hipEventCreate(&e0));
   EHIP(en, hipEventCreate(&e1));
-  // stage 1 reads U and its records and writes only stage buffers; a
-  // persistent launch needs its unit counter cleared first
-  auto one = [&](int gi) -> hipError_t {
-    hipError_t e = en->groups[gi].persistent ? reset_counters(en) : hipSuccess;
-    return e == hipSuccess ? enqueue_stage(en, 0, gi) : e;
+  // stage 1 reads U and its records and writes only stage buffers
+  auto one = [&](int gi) -> hipError_t { return enqueue_stage(en, 0, gi); };
+  auto timed = [&](int gi, float& ms) -> dgx_status {
+    EHIP(en, one(gi));
+    EHIP(en, hipEventRecord(e0, en->stream));
+    for (int r = 0; r < reps; ++r)
+      EHIP(en, one(gi));
+    EHIP(en, hipEventRecord(e1, en->stream));
+    EHIP(en, hipEventSynchronize(e1));
+    EHIP(en, hipEventElapsedTime(&ms, e0, e1));
+    return DGX_OK;
   };
-  for (int gi = 0; gi < (int)en->groups.size(); ++gi)
-  {
-    EnsGroup& g = en->groups[gi];
-    if (!active(en, g))
-      continue;
-    int best_w = g.waves, best_t = g.tile;
-    double best_ms = 0.0;
-    bool have = false;
-    for (int t = 0; t <= (g.persistent ? 1 : 0); ++t)
-      for (int w = g.persistent ? 2 : 1; w <= 3; ++w)
-      {
-        g.waves = w;
-        if (g.persistent)
-        {
-          g.tile = t;
-          en->planned = false;  // units and grid depend on the tile and target
-          dgx_status ps = plan(en, scheme, dt);
-          if (ps != DGX_OK)
-            return ps;
-        }
-        EHIP(en, one(gi));
-        EHIP(en, hipEventRecord(e0, en->stream));
-        for (int r = 0; r < reps; ++r)
-          EHIP(en, one(gi));
-        EHIP(en, hipEventRecord(e1, en->stream));
-        EHIP(en, hipEventSynchronize(e1));
-        float ms = 0.0f;
-        EHIP(en, hipEventElapsedTime(&ms, e0, e1));
-        if (!have || ms < best_ms)
-        {
-          best_ms = ms;
-          best_w = w;
-          best_t = t;
-          have = true;
-        }
-      }
-    g.waves = best_w;
-    g.tile  = best_t;
-    if (g.persistent)
+  auto replan = [&]() -> dgx_status {
+    en->planned = false;  // units and grid depend on the tile and target
+    return plan(en, scheme, dt);
+  };
+  // per launch group of the current layout: occupancy target and tile
+  auto tune_groups = [&]() -> dgx_status {
+    for (int gi = 0; gi < (int)en->groups.size(); ++gi)
     {
-      en->planned = false;
-      dgx_status ps = plan(en, scheme, dt);
-      if (ps != DGX_OK)
-        return ps;
+      EnsGroup& g = en->groups[gi];
+      if (!active(en, g))
+        continue;
+      int best_w = g.waves, best_t = g.tile;
+      float best_ms = 0.0f;
+      bool have = false;
+      for (int t = 0; t <= (g.unit_grid ? 1 : 0); ++t)
+        for (int w = g.unit_grid ? 2 : 1; w <= 3; ++w)
+        {
+          g.waves = w;
+          if (g.unit_grid)
+          {
+            g.tile = t;
+            dgx_status ps = replan();
+            if (ps != DGX_OK)
+              return ps;
+          }
+          float ms = 0.0f;
+          dgx_status ts = timed(gi, ms);
+          if (ts != DGX_OK)
+            return ts;
+          if (!have || ms < best_ms)
+          {
+            best_ms = ms;
+            best_w = w;
+            best_t = t;
+            have = true;
+          }
+        }
+      g.waves = best_w;
+      g.tile  = best_t;
+      if (g.unit_grid)
+      {
+        dgx_status ps = replan();
+        if (ps != DGX_OK)
+          return ps;
+      }
+    }
+    return DGX_OK;
+  };
+  // then the layout: the first stage of every active group, per layout
+  int best_layout = en->layout;
+  float best_ms = 0.0f;
+  bool have = false;
+  for (int layout : {1, 2, 0})
+  {
+    en->layout = layout;
+    dgx_status st2 = replan();
+    if (st2 == DGX_OK)
+      st2 = tune_groups();
+    float ms = 0.0f;
+    if (st2 == DGX_OK)
+      st2 = timed(-1, ms);
+    if (st2 != DGX_OK)
+      return st2;
+    if (!have || ms < best_ms)
+    {
+      best_ms = ms;
+      best_layout = layout;
+      have = true;
     }
   }
+  en->layout = best_layout;
+  if ((st = replan()) != DGX_OK)
+    return st;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (waves_out)
@@ -508,20 +518,20 @@ dgx_status dgx_ensemble_autotune(dgx_ensemble* en, int scheme, const double* dt,
   return dgx_ensemble_sync(en);
 }
 
-dgx_status dgx_ensemble_set_persistent(dgx_ensemble* en, int on)
+dgx_status dgx_ensemble_set_layout(dgx_ensemble* en, int layout)
 {
-  if (!en)
+  if (!en || layout < 0 || layout > 2)
     return DGX_ERR_INVALID;
-  if ((on != 0) != en->persistent)
+  if (layout != en->layout)
   {
     EHIP(en, hipStreamSynchronize(en->stream));
     drop_graph(en);
-    en->persistent = on != 0;
+    en->layout = layout;
   }
   return DGX_OK;
 }
 
-dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* en, int i, int* persistent, int* tile,
+dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* en, int i, int* layout, int* tile,
                                       int* waves)
 {
   if (!en || i < 0 || i >= (int)en->ctx.size())
@@ -529,10 +539,10 @@ dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* en, int i, int* persis
   for (const EnsGroup& g : en->groups)
     if (active(en, g) && std::find(g.members.begin(), g.members.end(), i) != g.members.end())
     {
-      if (persistent)
-        *persistent = g.persistent ? 1 : 0;
+      if (layout)
+        *layout = g.unit_grid ? en->layout : 0;
       if (tile)
-        *tile = g.persistent ? g.tile : 0;
+        *tile = g.unit_grid ? g.tile : 0;
       if (waves)
         *waves = g.waves;
       return DGX_OK;

@@ -1,16 +1,21 @@
-// dgx_ensemble_kernels.hip -- the persistent stage kernel of batched
-// ensembles (no counterpart in the reference, which runs one case per
-// process, dg.cpp / cudg.cu).
+// dgx_ensemble_kernels.hip -- the ensemble stage kernel of the one-launch
+// layouts (no counterpart in the reference, which runs one case per process,
+// dg.cpp / cudg.cu).
 //
 // One launch per RK stage serves every member of a q = 1 group whatever its
-// order: the kernel holds the stage bodies of p = 1, 2 and 3 (one-row or
-// tiled, a template choice) in one LDS region sized for the largest, and
-// each 64-lane workgroup loops taking the next work unit (one row, or one
-// tile, of one member's elements) from an atomic counter until every unit
-// is done.  Units are ordered by member, members by decreasing work, so the
-// large high-order members start first and the small ones fill the tail.
-// Every unit runs the arithmetic of the single-case kernels, so results are
-// those of dgx_step on each member to rounding.
+// order (PS = 0: the kernel holds the stage bodies of p = 1, 2 and 3, one-row
+// or tiled, a template choice, in one LDS region sized for the largest), or
+// every member of one order (PS = p: that order's body and LDS only).  The
+// grid is the concatenation of the members' work units (one row, or one
+// tile, of one member's elements), members in order of decreasing work, so
+// the in-order dispatch starts the large high-order members first and the
+// small ones fill the tail; each workgroup finds its member by a uniform
+// search of the item table.  Each workgroup runs one unit: a loop over units
+// (a persistent grid) keeps the body's uniform values live across
+// iterations and spills 50-70 scalar and ~100 vector registers at p = 3,
+// where the single pass needs 152 VGPRs and no spill.  Every unit runs the
+// arithmetic of the single-case kernels, so results are those of dgx_step
+// on each member to rounding.
 #include "dgx_stage.hpp"
 
 namespace dgx
@@ -44,47 +49,35 @@ __device__ __forceinline__ void ens_body(const BatchItem& it, int unit, int* sta
     stage_body_lds<P, 1, double, AFF>(it.mesh, it.ph, it.sa, it.lam, status, unit, lds);
 }
 
-template <int W, int TILE, bool AFF>
+// PS = 0: the bodies of every order in one kernel (LDS for the largest);
+// PS = 1..3: members of order PS only.  Block b runs unit b.
+template <int W, int TILE, bool AFF, int PS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
-void dgx_ens_stage_kernel(const BatchItem* __restrict__ items, int nitems, int nunits,
-                          int* __restrict__ counter, int* status)
+void dgx_ens_stage_kernel(const BatchItem* __restrict__ items, int nitems, int* status)
 {
-  __shared__ __attribute__((aligned(16))) unsigned char lds[ens_lds_bytes<TILE>()];
-  __shared__ int s_unit;
-  for (;;)
+  constexpr size_t LDS = PS ? ens_body_bytes<(PS ? PS : 1), TILE>() : ens_lds_bytes<TILE>();
+  __shared__ __attribute__((aligned(16))) unsigned char lds[LDS];
+  const int u = blockIdx.x;
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi)
   {
-    if (threadIdx.x == 0)
-      s_unit = atomicAdd(counter, 1);
-    __syncthreads();
-    const int u = s_unit;
-    __syncthreads();  // every lane has read s_unit before the next take
-    if (u >= nunits)
-      break;          // the counter only grows: every workgroup gets here
-    int lo = 0, hi = nitems - 1;
-    while (lo < hi)
-    {
-      const int mid = (lo + hi + 1) >> 1;
-      if (items[mid].block0 <= u)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const BatchItem& it = items[lo];
-    const int unit = u - it.block0;
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= u)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const BatchItem& it = items[lo];
+  const int unit = u - it.block0;
+  if constexpr (PS != 0)
+    ens_body<PS, TILE, AFF>(it, unit, status, lds);
+  else
     switch (it.p)
     {
       case 1: ens_body<1, TILE, AFF>(it, unit, status, lds); break;
       case 2: ens_body<2, TILE, AFF>(it, unit, status, lds); break;
       case 3: ens_body<3, TILE, AFF>(it, unit, status, lds); break;
     }
-    __syncthreads();  // the next unit reuses the LDS region
-  }
-}
-
-__global__ __launch_bounds__(64) void dgx_ens_reset_kernel(int* counters, int n)
-{
-  for (int i = threadIdx.x; i < n; i += 64)
-    counters[i] = 0;
 }
 
 int ens_units(int p, int nelem, int tile)
@@ -95,68 +88,50 @@ int ens_units(int p, int nelem, int tile)
   return (nelem + per - 1) / per;
 }
 
-template <int W, int TILE, bool AFF>
-static hipError_t launch_ens(const BatchItem* items, int nitems, int nunits, int* counter,
-                             int* status, int grid, hipStream_t st)
+template <int W, int TILE, bool AFF, int PS>
+static hipError_t launch_ens(const BatchItem* items, int nitems, int nunits, int* status,
+                             hipStream_t st)
 {
-  hipLaunchKernelGGL((dgx_ens_stage_kernel<W, TILE, AFF>), dim3(grid), dim3(64), 0, st, items,
-                     nitems, nunits, counter, status);
+  hipLaunchKernelGGL((dgx_ens_stage_kernel<W, TILE, AFF, PS>), dim3(nunits), dim3(64), 0, st,
+                     items, nitems, status);
   return hipGetLastError();
 }
 
-template <int TILE, bool AFF>
-static hipError_t launch_ens_w(const BatchItem* items, int nitems, int nunits, int* counter,
-                               int* status, int waves, int grid, hipStream_t st)
+template <int W, int TILE, bool AFF>
+static hipError_t launch_ens_p(const BatchItem* items, int nitems, int nunits, int* status,
+                               int psel, hipStream_t st)
 {
-  if (waves == 2)
-    return launch_ens<2, TILE, AFF>(items, nitems, nunits, counter, status, grid, st);
-  if (waves == 3)
-    return launch_ens<3, TILE, AFF>(items, nitems, nunits, counter, status, grid, st);
+  switch (psel)
+  {
+    case 0: return launch_ens<W, TILE, AFF, 0>(items, nitems, nunits, status, st);
+    case 1: return launch_ens<W, TILE, AFF, 1>(items, nitems, nunits, status, st);
+    case 2: return launch_ens<W, TILE, AFF, 2>(items, nitems, nunits, status, st);
+    case 3: return launch_ens<W, TILE, AFF, 3>(items, nitems, nunits, status, st);
+  }
   return hipErrorInvalidValue;
 }
 
-hipError_t ens_stage_dispatch(const BatchItem* items_dev, int nitems, int nunits, int* counter,
-                              int* status, int waves, int tile, bool affine, int grid,
-                              hipStream_t st)
-{
-  if (grid < 1 || nunits < 1)
-    return hipErrorInvalidValue;
-  if (tile)
-    return affine ? launch_ens_w<1, true>(items_dev, nitems, nunits, counter, status, waves,
-                                          grid, st)
-                  : launch_ens_w<1, false>(items_dev, nitems, nunits, counter, status, waves,
-                                           grid, st);
-  return affine ? launch_ens_w<0, true>(items_dev, nitems, nunits, counter, status, waves, grid,
-                                        st)
-                : launch_ens_w<0, false>(items_dev, nitems, nunits, counter, status, waves, grid,
-                                         st);
-}
-
-template <int W, int TILE, bool AFF>
-static int occ()
-{
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dgx_ens_stage_kernel<W, TILE, AFF>, 64,
-                                                   0) != hipSuccess)
-    return 0;
-  return n;
-}
-
-int ens_blocks_per_cu(int waves, int tile, bool affine)
+template <int TILE, bool AFF>
+static hipError_t launch_ens_w(const BatchItem* items, int nitems, int nunits, int* status,
+                               int waves, int psel, hipStream_t st)
 {
   if (waves == 2)
-    return tile ? (affine ? occ<2, 1, true>() : occ<2, 1, false>())
-                : (affine ? occ<2, 0, true>() : occ<2, 0, false>());
+    return launch_ens_p<2, TILE, AFF>(items, nitems, nunits, status, psel, st);
   if (waves == 3)
-    return tile ? (affine ? occ<3, 1, true>() : occ<3, 1, false>())
-                : (affine ? occ<3, 0, true>() : occ<3, 0, false>());
-  return 0;
+    return launch_ens_p<3, TILE, AFF>(items, nitems, nunits, status, psel, st);
+  return hipErrorInvalidValue;
 }
 
-hipError_t ens_reset_dispatch(int* counters, int n, hipStream_t st)
+hipError_t ens_stage_dispatch(const BatchItem* items_dev, int nitems, int nunits, int* status,
+                              int waves, int tile, bool affine, int psel, hipStream_t st)
 {
-  hipLaunchKernelGGL(dgx_ens_reset_kernel, dim3(1), dim3(64), 0, st, counters, n);
-  return hipGetLastError();
+  if (nunits < 1)
+    return hipErrorInvalidValue;
+  if (tile)
+    return affine ? launch_ens_w<1, true>(items_dev, nitems, nunits, status, waves, psel, st)
+                  : launch_ens_w<1, false>(items_dev, nitems, nunits, status, waves, psel, st);
+  return affine ? launch_ens_w<0, true>(items_dev, nitems, nunits, status, waves, psel, st)
+                : launch_ens_w<0, false>(items_dev, nitems, nunits, status, waves, psel, st);
 }
 
 hipError_t upload_tables_ensemble(const PTab* pt, const QTab* qt)

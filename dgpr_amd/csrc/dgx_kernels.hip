@@ -242,7 +242,6 @@ __global__ __launch_bounds__(64) void dgx_resid_out_kernel(MeshArgs mesh, const 
   const PTab& pt = c_ptab[P - 1];
   __shared__ double s[EPW][5][NB];
   __shared__ double sX[EPW][NG * 3];
-  __shared__ double red[64];
   int tid = threadIdx.x, le = tid / NB, node = tid % NB;
   int e0 = blockIdx.x * EPW, e = e0 + le;
   bool act = le < EPW && e < mesh.nelem;
@@ -301,38 +300,131 @@ __global__ __launch_bounds__(64) void dgx_resid_out_kernel(MeshArgs mesh, const 
   }
   if (partial)
   {
-    red[tid] = sq;
-    __syncthreads();
+    // butterfly over the wavefront (fixed order: deterministic)
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1)
+      sq += __shfl_xor(sq, w, 64);
     if (tid == 0)
-    {
-      double t = 0.0;
-      for (int x = 0; x < 64; ++x)
-        t += red[x];
-      partial[blockIdx.x] = t;
-    }
+      partial[blockIdx.x] = sq;
   }
+}
+
+// Per-workgroup sums of R^2, R = -B3^T W f_g (the modal residual of
+// dgx_resid_out_kernel), for the per-step residual norm: 4 x EPW elements
+// per 256-lane workgroup, the 1D tables copied to LDS (lane-dependent table
+// reads would otherwise be vector loads from constant memory), one butterfly
+// per wavefront.  partial[blockIdx.x] of norm_blocks(p, nelem) workgroups.
+template <int P, int Q>
+__global__ __launch_bounds__(256) void dgx_norm_kernel(MeshArgs mesh, const double* __restrict__ fg,
+                                                       double* __restrict__ partial)
+{
+  using Dm = Dims<P, Q>;
+  constexpr int N = Dm::N, NB = Dm::NB, NG = Dm::NG, EPB = 4 * Dm::EPW;
+  __shared__ double s[EPB][5][NB];
+  __shared__ double sX[EPB][NG * 3];
+  __shared__ PTab sPT;
+  __shared__ QTab sQT;
+  __shared__ double red[4];
+  const int tid = threadIdx.x, le = tid / NB, node = tid % NB;
+  const int e0 = blockIdx.x * EPB, e = e0 + le;
+  const bool act = le < EPB && e < mesh.nelem;
+  {
+    const double* a = reinterpret_cast<const double*>(&c_ptab[P - 1]);
+    double* b       = reinterpret_cast<double*>(&sPT);
+    for (int x = tid; x < (int)(sizeof(PTab) / 8); x += 256)
+      b[x] = a[x];
+    const double* c = reinterpret_cast<const double*>(&c_qtab[P - 1][Q - 1]);
+    double* d       = reinterpret_cast<double*>(&sQT);
+    for (int x = tid; x < (int)(sizeof(QTab) / 8); x += 256)
+      d[x] = c[x];
+  }
+  for (int x = tid; x < EPB * NG * 3; x += 256)
+  {
+    const int l = x / (NG * 3), ee = e0 + l;
+    sX[l][x % (NG * 3)] = (ee < mesh.nelem) ? mesh.nodes[(size_t)ee * NG * 3 + x % (NG * 3)] : 0.0;
+  }
+  double fgv[5];
+  if (act)
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+      fgv[r] = fg[((size_t)e * 5 + r) * NB + node];
+  __syncthreads();
+  const int ni = node % N, nj = (node / N) % N, nk = node / (N * N);
+  if (act)
+  {
+    double J[3][3];
+    jac_at<Q>(sQT, sX[le], -1, 0, ni, nj, nk, J);
+    const double wJ = sPT.w[ni] * sPT.w[nj] * sPT.w[nk] * det3(J);
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+      s[le][r][node] = -wJ * fgv[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    apply_dir<N, NB, EPB>(s, sPT.B, d, true, le, node, act);
+  double sq = 0.0;
+  if (act)
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+      sq += s[le][r][node] * s[le][r][node];
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1)
+    sq += __shfl_xor(sq, w, 64);
+  if ((tid & 63) == 0)
+    red[tid >> 6] = sq;
+  __syncthreads();
+  if (tid == 0)
+    partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // Deterministic sum of the per-block partial sums into out[0]; with
 // scale > 0, out[0] = sqrt(scale * sum) instead (the residual norm of
 // residual_norm, time_distributed.cpp:23-44, with scale = 1 / state length).
+// One workgroup: for up to SUM_ONE_BLOCK partials.
 __global__ __launch_bounds__(256) void dgx_sum_kernel(const double* partial, int n, double* out,
                                                       double scale)
 {
-  __shared__ double red[256];
+  __shared__ double red[4];
   double t = 0.0;
   for (int x = threadIdx.x; x < n; x += 256)
     t += partial[x];
-  red[threadIdx.x] = t;
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1)
+    t += __shfl_xor(t, w, 64);
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = t;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1)
-  {
-    if ((int)threadIdx.x < w)
-      red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
   if (threadIdx.x == 0)
-    out[0] = (scale > 0.0) ? sqrt(scale * red[0]) : red[0];
+  {
+    const double s = (red[0] + red[1]) + (red[2] + red[3]);
+    out[0] = (scale > 0.0) ? sqrt(scale * s) : s;
+  }
+}
+
+// First level of a long sum: workgroup b adds partial[b*SUM_CHUNK ..
+// (b+1)*SUM_CHUNK) into chunk[b] (eight independent loads per lane).
+__global__ __launch_bounds__(256) void dgx_chunk_sum_kernel(const double* __restrict__ partial,
+                                                            int n, double* __restrict__ chunk)
+{
+  __shared__ double red[4];
+  const int base = blockIdx.x * SUM_CHUNK + threadIdx.x;
+  double v[SUM_CHUNK / 256];
+#pragma unroll
+  for (int k = 0; k < SUM_CHUNK / 256; ++k)
+  {
+    const int x = base + 256 * k;
+    v[k] = (x < n) ? partial[x] : 0.0;
+  }
+  double t = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1)
+    t += __shfl_xor(t, w, 64);
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    chunk[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // Plain streaming copy (16 B per lane per iteration).
@@ -672,6 +764,29 @@ hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipS
 #undef CALL_LC
 }
 
+int norm_blocks(int p, int nelem)
+{
+  const int epb = 4 * (64 / ((p + 1) * (p + 1) * (p + 1)));
+  return (nelem + epb - 1) / epb;
+}
+
+template <int P, int Q>
+static hipError_t launch_norm(const MeshArgs& m, const double* fg, double* partial,
+                              hipStream_t st)
+{
+  hipLaunchKernelGGL((dgx_norm_kernel<P, Q>), dim3(norm_blocks(P, m.nelem)), dim3(256), 0, st, m,
+                     fg, partial);
+  return hipGetLastError();
+}
+
+hipError_t norm_partials_dispatch(int p, int q, const MeshArgs& m, const double* fg,
+                                  double* partial, hipStream_t st)
+{
+#define CALL_NM(P, Q) launch_norm<P, Q>(m, fg, partial, st)
+  DGX_DISPATCH_PQ(p, q, CALL_NM)
+#undef CALL_NM
+}
+
 hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg, double* f,
                               double* R, double* partial, hipStream_t st)
 {
@@ -830,17 +945,37 @@ hipError_t dot_dispatch(const double* x, const double* y, size_t n, double* part
   return sum_dispatch(partial, nblocks, out, st);
 }
 
+size_t sum_scratch(int n)
+{
+  return n > SUM_ONE_BLOCK ? (size_t)(n + SUM_CHUNK - 1) / SUM_CHUNK : 0;
+}
+
+// n partials at partial[0..n), then sum_scratch(n) doubles of scratch at
+// partial[n..): one pass of one workgroup, or chunk sums first
+static hipError_t reduce_dispatch(const double* partial, int n, double scale, double* out,
+                                  hipStream_t st)
+{
+  if (n <= SUM_ONE_BLOCK)
+  {
+    hipLaunchKernelGGL(dgx_sum_kernel, dim3(1), dim3(256), 0, st, partial, n, out, scale);
+    return hipGetLastError();
+  }
+  const int nc = (int)sum_scratch(n);
+  double* chunk = const_cast<double*>(partial) + n;
+  hipLaunchKernelGGL(dgx_chunk_sum_kernel, dim3(nc), dim3(256), 0, st, partial, n, chunk);
+  hipLaunchKernelGGL(dgx_sum_kernel, dim3(1), dim3(256), 0, st, chunk, nc, out, scale);
+  return hipGetLastError();
+}
+
 hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st)
 {
-  hipLaunchKernelGGL(dgx_sum_kernel, dim3(1), dim3(256), 0, st, partial, n, out, 0.0);
-  return hipGetLastError();
+  return reduce_dispatch(partial, n, 0.0, out, st);
 }
 
 hipError_t norm_dispatch(const double* partial, int n, double inv_count, double* out,
                          hipStream_t st)
 {
-  hipLaunchKernelGGL(dgx_sum_kernel, dim3(1), dim3(256), 0, st, partial, n, out, inv_count);
-  return hipGetLastError();
+  return reduce_dispatch(partial, n, inv_count, out, st);
 }
 
 hipError_t upload_tables(const PTab* pt, const QTab* qt)

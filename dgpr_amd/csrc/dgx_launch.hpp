@@ -109,21 +109,16 @@ hipError_t rhs_batch_dispatch(int p, int q, const BatchItem* items_dev, int nite
 // grid blocks of one mesh in a stage launch
 int stage_blocks(int p, int nelem);
 
-// ---- persistent ensemble stage kernel (dgx_ensemble_kernels.hip) ----------
-// One launch works through the units (tiles of elements) of every member of
-// a q = 1 group, of any order p = 1..3: each 64-lane workgroup takes the next
-// unit from *counter (atomic) until all nunits are done.  items_dev sorted
-// by first unit (block0), which ens_units counts per member.  tile 0: units
-// of one row (the one-row body); tile 1: units of the tiled body.  affine:
-// every member affine.  counter must read 0 at launch (ens_reset_dispatch).
+// ---- ensemble stage kernel of the one-launch layouts (dgx_ensemble_kernels.hip)
+// One launch covers the units (rows or tiles of elements) of every member of
+// a q = 1 group: block b runs unit b.  items_dev sorted by first unit
+// (block0), which ens_units counts per member.  tile 0: units of one row
+// (the one-row body); tile 1: units of the tiled body.  affine: every member
+// affine.  psel 0: members of any order (one kernel with every order's
+// body); psel 1..3: members of that order only.
 int ens_units(int p, int nelem, int tile);
-hipError_t ens_stage_dispatch(const BatchItem* items_dev, int nitems, int nunits, int* counter,
-                              int* status, int waves, int tile, bool affine, int grid,
-                              hipStream_t st);
-// resident workgroups per CU of the persistent kernel (occupancy query)
-int ens_blocks_per_cu(int waves, int tile, bool affine);
-// counters[0..n) = 0 with a kernel (no memset node in captured graphs)
-hipError_t ens_reset_dispatch(int* counters, int n, hipStream_t st);
+hipError_t ens_stage_dispatch(const BatchItem* items_dev, int nitems, int nunits, int* status,
+                              int waves, int tile, bool affine, int psel, hipStream_t st);
 hipError_t upload_tables_ensemble(const PTab* pt, const QTab* qt);
 // face trace records Tout of the Gauss-nodal state V (and the boundary
 // states of its domain-boundary faces, see trace_doubles)
@@ -138,6 +133,12 @@ hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg,
                               double* R, double* partial, hipStream_t st);
 hipError_t basis_dispatch(int p, const double* in, double* out, int nelem, int mode,
                           hipStream_t st);
+// Sums of n per-workgroup partials (deterministic order).  Above
+// SUM_ONE_BLOCK partials the sum runs in two launches (chunks of SUM_CHUNK,
+// then the chunk sums): the partial buffer then holds sum_scratch(n) more
+// doubles after the n partials.
+constexpr int SUM_ONE_BLOCK = 4096, SUM_CHUNK = 2048;
+size_t sum_scratch(int n);
 hipError_t sum_dispatch(const double* partial, int n, double* out, hipStream_t st);
 // *out = sqrt(inv_count * sum of the n partial sums): one step's residual norm
 hipError_t norm_dispatch(const double* partial, int n, double inv_count, double* out,
@@ -172,6 +173,11 @@ hipError_t mean_dispatch(const double* U, double* mean, size_t n, double old_ran
 hipError_t upload_tables(const PTab* pt, const QTab* qt);
 hipError_t copy_dispatch(const void* src, void* dst, size_t bytes, hipStream_t st);
 int resid_out_blocks(int p, int nelem);
+// per-workgroup sums of R^2 (R the modal residual of f_g) into
+// partial[0 .. norm_blocks(p, nelem)), for the per-step residual norm
+int norm_blocks(int p, int nelem);
+hipError_t norm_partials_dispatch(int p, int q, const MeshArgs& m, const double* fg,
+                                  double* partial, hipStream_t st);
 
 namespace host
 {

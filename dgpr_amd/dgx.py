@@ -41,8 +41,9 @@ BTYPES = {"periodic": PERIODIC, "freestream": FREESTREAM, "inviscid_wall": INVIS
 RK4, TVDRK3 = 0, 1
 SCHEMES = {"RK4": RK4, "tvdRK3": TVDRK3}
 PREC_FP64, PREC_MIXED = 0, 1
-PREC_MIXED_BF16 = 2
-PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "mixed_bf16": PREC_MIXED_BF16}
+PREC_MIXED_BF16, PREC_MIXED_RECORDS = 2, 3
+PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "mixed_bf16": PREC_MIXED_BF16,
+              "mixed_records": PREC_MIXED_RECORDS}
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "HIP error",
           4: "NaN residual norm", 5: "I/O error", 6: "no HIP device"}
@@ -130,7 +131,7 @@ def lib():
         L.dgx_ensemble_autotune.argtypes = [C.c_void_p, C.c_int, PD, C.c_int,
                                             C.POINTER(C.c_int)]
         L.dgx_ensemble_last_error.argtypes = [C.c_void_p]
-        L.dgx_ensemble_set_persistent.argtypes = [C.c_void_p, C.c_int]
+        L.dgx_ensemble_set_layout.argtypes = [C.c_void_p, C.c_int]
         L.dgx_ensemble_member_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.dgx_ensemble_last_error.restype = C.c_char_p
@@ -168,7 +169,7 @@ def lib():
                    "dgx_read_dg_output",
                    "dgx_ensemble_create", "dgx_ensemble_destroy", "dgx_ensemble_member",
                    "dgx_ensemble_step", "dgx_ensemble_sync", "dgx_ensemble_get_stream",
-                   "dgx_ensemble_autotune", "dgx_ensemble_set_persistent",
+                   "dgx_ensemble_autotune", "dgx_ensemble_set_layout",
                    "dgx_ensemble_member_tuning", "dgx_kinetic_energy", "dgx_residual_vjp",
                    "dgx_adjoint_rk4", "dgx_adjoint_release", "dgx_residual_vjp_device_noalloc"):
             getattr(L, nm).restype = st
@@ -517,6 +518,8 @@ class Solver:
                  "Roe, pressure/energy, viscous physics, inverse mass, RK in FP64",
         "mixed_bf16": "as mixed, the contractions as 3-term split-BF16 products "
                       "(v_mfma_f32_4x4x4bf16_1k, FP32 accumulation)",
+        "mixed_records": "FP32 face trace records only; contractions, geometry, fluxes, "
+                         "Roe, inverse mass, RK in FP64",
     }
 
     # rows per tile of the tiled stage kernel at p = 1, 2, 3
@@ -622,13 +625,15 @@ class Ensemble:
         self._chk(lib().dgx_ensemble_autotune(self.h, sc, _p(dts), int(reps), w))
         return list(w)
 
-    def set_persistent(self, on):
-        """True: one persistent launch per stage for the q = 1 members
-        (default); False: one batched launch per (p, q, affine) group."""
-        self._chk(lib().dgx_ensemble_set_persistent(self.h, 1 if on else 0))
+    def set_layout(self, layout):
+        """Layout of the q = 1 members' launches: 1 one launch per stage for
+        all orders (default); 2 one launch per order; 0 one batched launch
+        per (p, q, affine) group."""
+        self._chk(lib().dgx_ensemble_set_layout(self.h, int(layout)))
 
     def member_tuning(self, i):
-        """(persistent, tile, waves) of the launch member i runs in."""
+        """(layout, tile, waves) of the launch member i runs in (layout 0:
+        a batched launch)."""
         p, t, w = C.c_int(), C.c_int(), C.c_int()
         self._chk(lib().dgx_ensemble_member_tuning(self.h, int(i), C.byref(p), C.byref(t),
                                                    C.byref(w)))

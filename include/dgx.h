@@ -217,17 +217,18 @@ dgx_status dgx_set_mean(dgx_ctx* ctx, const double* mean_host);
  * case per process, dg.cpp / cudg.cu) ----------------------------------------
  * An ensemble owns one context per member (meshes of any size, p and q; own
  * parameters).  dgx_ensemble_step advances every member by nsteps steps of
- * its own dt.  Per RK stage, by default, all q = 1 members of every order run
- * as ONE launch of a persistent kernel whose workgroups take work units
- * (rows or tiles of one member's elements, largest members first) from an
- * atomic counter (affine and curved members form two such launches; q > 1
- * members run one batched launch per (p, q)); with
- * dgx_ensemble_set_persistent(ens, 0) every (p, q, affine) group runs its
- * own batched launch instead.  With use_graph != 0 one step of the whole
- * ensemble is captured once in a hipGraph and replayed.  Results equal
- * those of dgx_step on each member (the same arithmetic, to rounding) in
- * both modes.  The call only
- * enqueues work on the ensemble's stream: call dgx_ensemble_sync before
+ * its own dt.  Per RK stage, by default (layout 1), all q = 1 members of
+ * every order run as ONE launch of the ensemble stage kernel over the
+ * concatenation of their work units (rows or tiles of one member's elements,
+ * largest members first; affine and curved members form two such launches;
+ * q > 1 members run one batched launch per (p, q)); with
+ * dgx_ensemble_set_layout(ens, 2) that kernel runs once per (order, affine)
+ * class of the q = 1 members instead, and with dgx_ensemble_set_layout(ens,
+ * 0) every (p, q, affine) group runs its own batched launch of the one-row
+ * kernel.  With use_graph != 0 one step of the whole ensemble is captured
+ * once in a hipGraph and replayed.  Results equal those of dgx_step on each
+ * member (the same arithmetic, to rounding) in every
+ * layout.  The call only enqueues work on the ensemble's stream: call dgx_ensemble_sync before
  * using a member context again. */
 typedef struct dgx_ensemble dgx_ensemble;
 dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_params* params,
@@ -241,21 +242,26 @@ dgx_status dgx_ensemble_step(dgx_ensemble* ens, int scheme, const double* dt, ui
  * subsonic inflow without a Mach root. */
 dgx_status dgx_ensemble_sync(dgx_ensemble* ens);
 dgx_status dgx_ensemble_get_stream(dgx_ensemble* ens, void** stream);
-/* Times every launch group of the current mode at each occupancy target and,
- * for the persistent kernel, each tile (one-row units: 64/(p+1)^3 elements
- * per workgroup, one node per lane; tiled units: 2, 6 or 8 elements per
- * workgroup at p = 3, 2, 1, up to 3 nodes per lane), with reps launches of
- * the first stage (which writes stage buffers only), keeps the fastest per
- * group and writes each member's occupancy target to waves_out[ncase] (may
- * be NULL; dgx_ensemble_member_tuning reads the full choice). */
+/* For each layout (1, 2, 0) times every launch group at each occupancy
+ * target and, for the ensemble stage kernel,
+ * each tile (one-row units: 64/(p+1)^3 elements per workgroup, one node per
+ * lane; tiled units: 2, 6 or 8 elements per workgroup at p = 3, 2, 1, up to
+ * 3 nodes per lane), with reps launches of the first stage (which writes
+ * stage buffers only), keeps the fastest per group, then keeps the layout
+ * whose first stage over all members is fastest (a later
+ * dgx_ensemble_set_layout overrides it).  Writes each member's
+ * occupancy target to waves_out[ncase] (may be NULL;
+ * dgx_ensemble_member_tuning reads the full choice). */
 dgx_status dgx_ensemble_autotune(dgx_ensemble* ens, int scheme, const double* dt, int reps,
                                  int* waves_out);
-/* on != 0 (the default): the persistent single launch per stage for q = 1
- * members; 0: one batched launch per (p, q, affine) group. */
-dgx_status dgx_ensemble_set_persistent(dgx_ensemble* ens, int on);
-/* The launch member i runs in: persistent (0/1), tile (0/1) and occupancy
+/* Layout of the q = 1 members' launches: 1 (the default) one launch per
+ * stage for all orders; 2 one launch per order; 0 one batched launch per
+ * (p, q, affine) group. */
+dgx_status dgx_ensemble_set_layout(dgx_ensemble* ens, int layout);
+/* The launch member i runs in: layout (0 for a batched launch, else the
+ * layout 1 or 2 of dgx_ensemble_set_layout), tile (0/1) and occupancy
  * target (waves per SIMD). */
-dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* ens, int i, int* persistent, int* tile,
+dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* ens, int i, int* layout, int* tile,
                                       int* waves);
 const char* dgx_ensemble_last_error(const dgx_ensemble* ens);
 
@@ -289,17 +295,20 @@ dgx_status dgx_residual_vjp_device_noalloc(dgx_ctx* ctx, const double* w_dev, do
  * ends at the final state, as after dgx_step) and returns the final kinetic
  * energy J, its gradient with respect to the initial state (simstate layout,
  * may be NULL) and dJ/dmu (may be NULL), by the discrete adjoint.
- * interval >= 1: a checkpoint every `interval` steps, each segment
- * recomputed once in the reverse sweep.  interval = 0: every stage state
- * with its trace records (and f, for dJ/dmu) is kept from the forward sweep
- * when nsteps * 4 * (state + records [+ state]) fits in three quarters of
- * the free device memory, so nothing is recomputed; otherwise checkpoints
- * every ceil(sqrt(nsteps)) steps.  Both schedules give the same gradients
- * bit for bit.  The stored stages stay allocated on the context for later
- * calls (a buffer that is large enough is reused; freeing tens of GB can
- * take seconds) until dgx_adjoint_release or dgx_destroy; the free-memory
- * test counts the held buffer as free, and a call that checkpoints frees a
- * held buffer before its own allocations. */
+ * interval >= 1: a checkpoint every `interval` steps; in the reverse sweep
+ * each segment is replayed once into a store of `interval` steps' stage
+ * states with their trace records (and f, for dJ/dmu), or, when that store
+ * does not fit, its step-start states are recomputed and each step's stages
+ * rebuilt.  interval = 0: every stage state with its records is kept from
+ * the forward sweep when nsteps * 4 * (state + records [+ state]) fits in
+ * three quarters of the free device memory, so nothing is recomputed;
+ * otherwise checkpoints every ceil(sqrt(nsteps)) steps as above.  All
+ * schedules give the same gradients bit for bit.  The store stays allocated
+ * on the context for later calls (a buffer that is large enough is reused;
+ * freeing tens of GB can take seconds) until dgx_adjoint_release or
+ * dgx_destroy; the free-memory test counts the held buffer as free, and a
+ * call that checkpoints frees a held buffer too small for its segments
+ * before its own allocations. */
 dgx_status dgx_adjoint_rk4(dgx_ctx* ctx, double dt, uint64_t nsteps, uint64_t interval,
                            double* J, double* dJdU0_host, double* dJdmu);
 
@@ -316,8 +325,10 @@ typedef enum dgx_precision
   DGX_PREC_FP64       = 0, /* everything in FP64 (default)                  */
   DGX_PREC_MIXED      = 1, /* FP32 trace records; p = 3 tiled kernel: volume
                               contractions in FP32 on the matrix cores       */
-  DGX_PREC_MIXED_BF16 = 2  /* as MIXED, the contractions as split-BF16
+  DGX_PREC_MIXED_BF16 = 2, /* as MIXED, the contractions as split-BF16
                               (3-term) products on the BF16 matrix cores    */
+  DGX_PREC_MIXED_RECORDS = 3 /* FP32 trace records only; every contraction
+                              in FP64                                       */
 } dgx_precision;
 
 /* The mixed forms store the face trace records that cross element faces
@@ -327,7 +338,10 @@ typedef enum dgx_precision
  * momentum, and the projection of the contravariant volume flux onto the
  * nodes) on the matrix cores with FP32 accumulation: FP32 operands
  * (DGX_PREC_MIXED) or each FP32 operand split into three BF16 terms, the six
- * leading products accumulated (DGX_PREC_MIXED_BF16).  Geometry, the Roe
+ * leading products accumulated (DGX_PREC_MIXED_BF16).  DGX_PREC_MIXED_RECORDS
+ * keeps the FP32 records and runs every contraction in FP64 (at p = 3 the
+ * fastest and most accurate of the three: the contractions are a small part
+ * of the kernel and the matrix-core forms add conversions).  Geometry, the Roe
  * flux and its eigen-structure, pressure/energy recovery, the viscous flux
  * physics, the inverse mass and the RK accumulation stay FP64.  Changing the
  * form drops the stored records (the next stage recomputes them).  The

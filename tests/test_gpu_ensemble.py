@@ -1,9 +1,10 @@
 """Batched ensembles: members of different size, p, q, boundary conditions
-and parameters advanced together (one batched launch per (p, q, affine)
-group and RK stage, optionally replayed from a captured hipGraph) must
-reproduce the single-case stepper on each member, and the oracle.  The
-members include affine and curved q = 1 meshes of the same p = 2, which
-form two groups (constant-Jacobian and general kernels)."""
+and parameters advanced together (in every layout: one launch per stage for
+all orders, one per order, or one batched launch per (p, q, affine) group;
+optionally replayed from a captured hipGraph) must reproduce the
+single-case stepper on each member, and the oracle.  The members include
+affine and curved q = 1 meshes of the same p = 2, which form two groups
+(constant-Jacobian and general kernels)."""
 import numpy as np
 import pytest
 
@@ -37,7 +38,7 @@ def built():
 def single_runs(cs, scheme, nsteps, tiles=None):
     """Single-case steps of every member with the stage tile the ensemble
     runs it with (tiles[i]; by default 0, the one-row kernel of the batched
-    launches and of the persistent kernel before tuning)."""
+    launches and of the ensemble stage kernel before tuning)."""
     out = []
     for i, c in enumerate(cs):
         with dgx.Solver(c.mesh, c.params) as s:
@@ -48,18 +49,20 @@ def single_runs(cs, scheme, nsteps, tiles=None):
     return out
 
 
-@pytest.mark.parametrize("scheme,persistent", [("RK4", True), ("tvdRK3", True), ("RK4", False),
-                                               ("tvdRK3", False)])
-def test_ensemble_matches_single_cases(built, scheme, persistent):
+@pytest.mark.parametrize("scheme,layout", [("RK4", 1), ("tvdRK3", 1), ("RK4", 2), ("RK4", 0),
+                                           ("tvdRK3", 0)])
+def test_ensemble_matches_single_cases(built, scheme, layout):
+    """Every layout: one launch for all orders (1), one per order
+    (2), batched launches per (p, q, affine) (0)."""
     cs = members()
     nsteps = 3
     ref = single_runs(cs, scheme, nsteps)
     dts = [c.dt for c in cs]
     with dgx.Ensemble([c.mesh for c in cs], [c.params for c in cs]) as en:
-        en.set_persistent(persistent)
+        en.set_layout(layout)
         for i, c in enumerate(cs):
             p, t, w = en.member_tuning(i)
-            assert p == (1 if (persistent and c.mesh.q == 1) else 0)
+            assert p == (layout if c.mesh.q == 1 else 0)
         for m, c in zip(en.members, cs):
             m.set_state(c.U0)
         en.step(scheme, dts, nsteps, graph=False)
@@ -95,7 +98,7 @@ def test_ensemble_member_matches_oracle(orc, built):
 
 
 def test_ensemble_autotune_keeps_results(built):
-    """The autotuner (occupancy targets; tiles of the persistent kernel)
+    """The autotuner (occupancy targets; tiles of the ensemble stage kernel)
     leaves the states alone, and its choice gives the single-case results
     of the same tile to rounding."""
     cs = members()
@@ -108,6 +111,9 @@ def test_ensemble_autotune_keeps_results(built):
         print("  autotuned waves per member", w, "tuning", tun, flush=True)
         assert all(x in (1, 2, 3) for x in w)
         assert all(t[1] in (0, 1) for t in tun)
+        # one layout for all q = 1 members; q > 1 members always batched
+        assert len({t[0] for t, c in zip(tun, cs) if c.mesh.q == 1}) == 1
+        assert all(t[0] == 0 for t, c in zip(tun, cs) if c.mesh.q != 1)
         en.step("RK4", dts, 2, graph=True)
         got = [m.get_state() for m in en.members]
     ref = single_runs(cs, "RK4", 2, tiles=[t[1] for t in tun])
