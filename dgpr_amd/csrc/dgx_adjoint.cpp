@@ -281,7 +281,7 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
     AHIP(c, alloc(tz, ntr));
     AHIP(c, alloc(f0, n));
   }
-  const int ndot = 256;
+  const int ndot = 1024;
   AHIP(c, alloc(dpart, ndot));
   // one dot product per reverse stage for dJ/dmu, read back once at the end
   AHIP(c, alloc(dsum, dJdmu ? 4 * nsteps : 1));
@@ -366,8 +366,8 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
         AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), ph0, Si[i], tz.p, c->status,
                                c->stream));
         AHIP(c, stage(c, ph0, Si[i], tz.p, nullptr, nullptr, nullptr, f0.p, 0.0));
-        AHIP(c, axpby_dispatch(f0.p, Ki[i], 1.0 / c->phys.mu, -1.0 / c->phys.mu, n, c->stream));
-        AHIP(c, dot_dispatch(lk.p, f0.p, n, dpart.p, ndot, dsum.p + ndmu, c->stream));
+        AHIP(c, dot_dispatch(lk.p, Ki[i], f0.p, 1.0 / c->phys.mu, n, dpart.p, ndot,
+                             dsum.p + ndmu, c->stream));
         ++ndmu;
       }
       AHIP(c, adjoint_dispatch(c->p, c->q, mesh_args(c), c->phys, Si[i], Ti[i], c->lam, lk.p,

@@ -22,15 +22,16 @@
 // PART 0: the volume terms (into pvol); PART 1: the face terms (into aown,
 // anbr).  Separate launches: the volume part needs about half the registers
 // of the face part and runs at a higher occupancy on its own.
+// Occupancy target: at q = 1 the face-term part holds 256 VGPRs + 14-24
+// AGPRs, one wave per SIMD; asked for two it spills 10 VGPRs (44 B/lane at
+// p = 2) and runs the state-gradient adjoint 6 % faster (same results bit for
+// bit).  q > 1 spills already at one wave and keeps it.
 template <int P, int Q, int PART, bool AFF>
-__global__ __launch_bounds__(64) void dgx_adj_main_kernel(MeshArgs mesh, Phys ph,
-                                                          const double* __restrict__ V,
-                                                          const double* __restrict__ Tin,
-                                                          const double* __restrict__ lam,
-                                                          const double* __restrict__ psi,
-                                                          double* __restrict__ pvol,
-                                                          double* __restrict__ aown,
-                                                          double* __restrict__ anbr)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Q == 1 ? 2 : 1)))
+void dgx_adj_main_kernel(MeshArgs mesh, Phys ph, const double* __restrict__ V,
+                         const double* __restrict__ Tin, const double* __restrict__ lam,
+                         const double* __restrict__ psi, double* __restrict__ pvol,
+                         double* __restrict__ aown, double* __restrict__ anbr)
 {
   using FD = FaceDims<P>;
   constexpr int N = FD::N, NB = FD::NB, NF = FD::NF, EPW = FD::EPW;
@@ -725,16 +726,6 @@ __global__ __launch_bounds__(64) void dgx_kinetic_energy_kernel(MeshArgs mesh,
     partial[blockIdx.x] = red[0];
 }
 
-// y = a x + b y
-__global__ __launch_bounds__(256) void dgx_axpby_kernel(double* __restrict__ y,
-                                                        const double* __restrict__ x, double a,
-                                                        double b, size_t n)
-{
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n)
-    y[i] = a * x[i] + b * y[i];
-}
-
 // Tail of one reverse RK stage, elementwise in one pass: with the stage's
 // adjoint a, sum_out = sum_in + a (sum_in may be sum_out) and the next
 // stage's input lk = wa a + wl l; without a, lk = wl l.  Null outputs are
@@ -760,23 +751,23 @@ __global__ __launch_bounds__(256) void dgx_adj_tail_kernel(const double* __restr
     lk[i] = wl * l[i];
 }
 
-// per-block partial sums of x . y (256 threads, grid-stride)
+// partial[b] = scale * sum over workgroup b's grid-stride share of
+// x[i] * (y[i] - z[i]) (z may be NULL: x . y); wave butterflies, fixed order
 __global__ __launch_bounds__(256) void dgx_dot_kernel(const double* __restrict__ x,
-                                                      const double* __restrict__ y, size_t n,
-                                                      double* __restrict__ partial)
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z, double scale,
+                                                      size_t n, double* __restrict__ partial)
 {
-  __shared__ double red[256];
+  __shared__ double red[4];
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
-    s += x[i] * y[i];
-  red[threadIdx.x] = s;
+    s += x[i] * (z ? y[i] - z[i] : y[i]);
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1)
+    s += __shfl_xor(s, w, 64);
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1)
-  {
-    if ((int)threadIdx.x < o)
-      red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
   if (threadIdx.x == 0)
-    partial[blockIdx.x] = red[0];
+    partial[blockIdx.x] = scale * ((red[0] + red[1]) + (red[2] + red[3]));
 }

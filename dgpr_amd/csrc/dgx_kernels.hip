@@ -917,14 +917,6 @@ hipError_t kinetic_energy_dispatch(int p, int q, const MeshArgs& m, const double
 #undef CALL_KE
 }
 
-hipError_t axpby_dispatch(double* y, const double* x, double a, double b, size_t n,
-                          hipStream_t st)
-{
-  int nblk = (int)((n + 255) / 256);
-  hipLaunchKernelGGL(dgx_axpby_kernel, dim3(nblk), dim3(256), 0, st, y, x, a, b, n);
-  return hipGetLastError();
-}
-
 hipError_t adj_tail_dispatch(const double* l, const double* a, const double* sum_in,
                              double* sum_out, double* lk, double wl, double wa, size_t n,
                              hipStream_t st)
@@ -935,10 +927,10 @@ hipError_t adj_tail_dispatch(const double* l, const double* a, const double* sum
   return hipGetLastError();
 }
 
-hipError_t dot_dispatch(const double* x, const double* y, size_t n, double* partial,
-                        int nblocks, double* out, hipStream_t st)
+hipError_t dot_dispatch(const double* x, const double* y, const double* z, double scale,
+                        size_t n, double* partial, int nblocks, double* out, hipStream_t st)
 {
-  hipLaunchKernelGGL(dgx_dot_kernel, dim3(nblocks), dim3(256), 0, st, x, y, n, partial);
+  hipLaunchKernelGGL(dgx_dot_kernel, dim3(nblocks), dim3(256), 0, st, x, y, z, scale, n, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;

@@ -158,12 +158,10 @@ hipError_t kinetic_energy_dispatch(int p, int q, const MeshArgs& m, const double
 hipError_t adj_tail_dispatch(const double* l, const double* a, const double* sum_in,
                              double* sum_out, double* lk, double wl, double wa, size_t n,
                              hipStream_t st);
-// y = a x + b y
-hipError_t axpby_dispatch(double* y, const double* x, double a, double b, size_t n,
-                          hipStream_t st);
-// *out = x . y (nblocks partial sums in partial)
-hipError_t dot_dispatch(const double* x, const double* y, size_t n, double* partial,
-                        int nblocks, double* out, hipStream_t st);
+// *out = scale * x . (y - z) (z may be NULL: scale * x . y); nblocks
+// partial sums in partial (nblocks <= SUM_ONE_BLOCK)
+hipError_t dot_dispatch(const double* x, const double* y, const double* z, double scale,
+                        size_t n, double* partial, int nblocks, double* out, hipStream_t st);
 // derived outputs of a Gauss-nodal state on the (Ns+1)^3 sub-grid of each element
 hipError_t sample_dispatch(int p, const double* V, int nelem, int Ns, int var, double gamma,
                            double* out, hipStream_t st);
