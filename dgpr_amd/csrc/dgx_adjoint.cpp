@@ -305,6 +305,8 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   const uint64_t K = interval ? interval : (uint64_t)std::ceil(std::sqrt((double)nsteps));
   auto acquire_store = [&](uint64_t steps) -> double* {
     const size_t need = (size_t)steps * per_step * sizeof(double);
+    if (c->adj_store_limit && need > c->adj_store_limit)
+      return nullptr;
     if (c->adj_store && c->adj_store_bytes >= need)
       return c->adj_store;
     size_t fr = 0, tot = 0;
@@ -571,6 +573,16 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
     *J = Jv;
   if (dJdmu)
     *dJdmu = dmu_total;
+  return DGX_OK;
+}
+
+dgx_status dgx_adjoint_set_store_limit(dgx_ctx* c, uint64_t bytes)
+{
+  if (!c)
+    return DGX_ERR_INVALID;
+  c->adj_store_limit = (size_t)bytes;
+  if (bytes && c->adj_store && c->adj_store_bytes > bytes)
+    return dgx_adjoint_release(c);
   return DGX_OK;
 }
 

@@ -34,11 +34,12 @@ struct dgx_ctx
   bool tu_valid;    // TU holds the traces of the current U
   int stage_waves;  // occupancy target of the stage kernel (1..3)
   int stage_tile;   // 0: one-row stage kernel, 1: tiled stage kernel
-  int prec;         // DGX_PREC_FP64 or DGX_PREC_MIXED (float trace records)
+  int prec;         // DGX_PREC_* (float trace records in the mixed forms)
   int affine;       // q = 1 and every element a parallelepiped
   double* mean;     // running mean (Gauss-nodal), allocated on first use
   double* adj_store;       // stored stages of the last dgx_adjoint_rk4 (kept:
   size_t adj_store_bytes;  // freeing tens of GB can take seconds)
+  size_t adj_store_limit;  // most bytes that store may take (0: no limit)
   int* status;      // kernel-side error flags
   int npartial;
   size_t bytes;

@@ -120,6 +120,7 @@ def lib():
         L.dgx_residual_vjp.argtypes = [P_ctx, PD, PD]
         L.dgx_adjoint_rk4.argtypes = [P_ctx, D, U64, U64, PD, PD, PD]
         L.dgx_adjoint_release.argtypes = [P_ctx]
+        L.dgx_adjoint_set_store_limit.argtypes = [P_ctx, U64]
         L.dgx_residual_vjp_device_noalloc.argtypes = [P_ctx, C.c_void_p, C.c_void_p]
         L.dgx_ensemble_create.argtypes = [C.c_int, C.POINTER(_CMesh), C.POINTER(Params),
                                           C.POINTER(C.c_void_p)]
@@ -171,7 +172,8 @@ def lib():
                    "dgx_ensemble_step", "dgx_ensemble_sync", "dgx_ensemble_get_stream",
                    "dgx_ensemble_autotune", "dgx_ensemble_set_layout",
                    "dgx_ensemble_member_tuning", "dgx_kinetic_energy", "dgx_residual_vjp",
-                   "dgx_adjoint_rk4", "dgx_adjoint_release", "dgx_residual_vjp_device_noalloc"):
+                   "dgx_adjoint_rk4", "dgx_adjoint_release", "dgx_adjoint_set_store_limit",
+                   "dgx_residual_vjp_device_noalloc"):
             getattr(L, nm).restype = st
         _lib = L
     return _lib
@@ -478,6 +480,11 @@ class Solver:
     def adjoint_release(self):
         """Free the stored-stage buffer adjoint_rk4 keeps between calls."""
         _chk(lib().dgx_adjoint_release(self.ctx), self.ctx)
+
+    def adjoint_store_limit(self, nbytes):
+        """Caps the stored-stage buffer of adjoint_rk4 at nbytes (0: no cap);
+        a schedule whose store would exceed it falls to the next one."""
+        _chk(lib().dgx_adjoint_set_store_limit(self.ctx, int(nbytes)), self.ctx)
 
     def sample_output(self, var, N):
         """eval_output (render/vrtxgen.cpp:25-59) of the current state on the

@@ -316,6 +316,13 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* ctx, double dt, uint64_t nsteps, uint64_t in
  * (no counterpart in the reference, which has no adjoint).  No-op when none
  * is held. */
 dgx_status dgx_adjoint_release(dgx_ctx* ctx);
+/* Caps the stored-stage buffer of dgx_adjoint_rk4 at `bytes` (0: no cap
+ * beyond three quarters of the free device memory; no counterpart in the
+ * reference).  A schedule whose store would exceed the cap falls to the next
+ * one (every stage stored -> one replay per checkpoint segment -> stages
+ * rebuilt per step); a held buffer above the cap is freed.  Gradients are
+ * the same bit for bit on every schedule. */
+dgx_status dgx_adjoint_set_store_limit(dgx_ctx* ctx, uint64_t bytes);
 
 /* ---- precision form (no counterpart in the reference: its CPU solver is
  * FP64 and its CUDA driver FP32 throughout, cudg.cu:26-27) ----------------- */

@@ -97,13 +97,22 @@ def test_adjoint_rk4_gradient_matches_fd(built, name, nsteps, interval):
 
 @pytest.mark.parametrize("name", ["tgv2_p2_curved", "tgv3_p1"])
 def test_adjoint_schedules_agree_bitwise(built, name):
-    """Stored stages (interval 0, which fits here), every step-start state
-    kept (1) and checkpoints every 3 steps run the same kernels on the same
-    inputs: J, both gradients and the final state agree bit for bit."""
+    """Every schedule runs the same kernels on the same inputs: every stage
+    stored (interval 0); checkpoints every step and every 3 steps with one
+    replay per segment into a segment store; interval 0 under a store cap
+    below the full store (falls to ceil(sqrt(5)) = 3-step segments);
+    checkpoints every 3 steps under a cap below one segment (each step's
+    stages rebuilt).  J, both gradients and the final state agree bit for
+    bit."""
     c = small_cases()[name]
+    m = c.mesh
+    n = c.U0.size
+    ntr = m.nelem * 6 * 8 * (m.p + 1) ** 2  # periodic: no boundary states
+    full = 5 * (4 * (n + ntr) + 4 * n) * 8  # bytes of every stage stored, with dJ/dmu
     out = []
-    for interval in (0, 1, 3):
+    for interval, cap in ((0, 0), (1, 0), (3, 0), (0, full - 8), (3, 1)):
         with dgx.Solver(c.mesh, c.params) as s:
+            s.adjoint_store_limit(cap)
             s.set_state(c.U0)
             J, g, dmu = s.adjoint_rk4(c.dt, 5, interval)
             out.append((J, g, dmu, s.get_state()))

@@ -96,12 +96,12 @@ CASES = {
 }
 
 
-# Solution-only cases here.  Long horizon: 100 RK4 steps at the CFL
-# (dt c / h) of the 64^3 workload.  Its residual is compared in
-# tests/test_gpu_reference_golden.py against the reference's own residual
-# and the exact (extended-precision) value of the discrete residual: at
-# Mach 0.1 the reference's double f is itself 1.5e-12 from exact, so the
-# bound there is relative to that rounding floor.
+# Long horizon: 100 RK4 steps at the CFL (dt c / h) of the 64^3 workload.
+# Its residual is bounded relative to its rounding floor
+# (test_long_case_residual_at_the_rounding_floor); the same physics is
+# compared in tests/test_gpu_reference_golden.py against the reference's own
+# residual and the exact (extended-precision) value of the discrete residual:
+# at Mach 0.1 the reference's double f is itself 1.5e-12 from exact.
 LONG_CASES = {
     "tgv4_p3_long": lambda: cases.tgv_case(n=4, p=3, seed=25, dt=4e-3, nsteps=100),
 }
@@ -129,6 +129,30 @@ def test_residual_parity(name, orc, built):
     print("  rel err f %.3e  R %.3e" % (ef, eR), flush=True)
     assert ef < RES_TOL
     assert eR < RES_TOL
+
+
+def test_long_case_residual_at_the_rounding_floor(orc, built):
+    """tgv4_p3_long (p = 3, Mach 0.1: the physics of the 64^3 workload): the
+    device residual against the oracle, bounded by the case's own rounding
+    floor, measured here on the CPU: the oracle's f moves by `floor`
+    (relative 2-norm) when the state changes by 1e-15 relative, far more
+    than 1e-15 because f is a small difference of large fluxes at Mach 0.1.
+    Bar: max(1e-12, 2 floor)."""
+    case = LONG_CASES["tgv4_p3_long"]()
+    m = case.mesh
+    geo = orc.Geometry(m.p, m.q, m.nx, m.ny, m.nz, m.bounds, m.nodes)
+    prm = orc_params(orc, case.params)
+    _, f_o = geo.residual(prm, case.U0)
+    U1 = case.U0 * (1.0 + 1e-15 * np.random.default_rng(7).uniform(-1.0, 1.0, case.U0.shape))
+    _, f_1 = geo.residual(prm, U1)
+    floor = rel(f_1, f_o)
+    with dgx.Solver(m, case.params) as s:
+        s.set_state(case.U0)
+        R_g, f_g = s.residual()
+    ef = rel(f_g, f_o)
+    print("  tgv4_p3_long: device vs oracle f %.3e; oracle f under a 1e-15 state change %.3e"
+          % (ef, floor), flush=True)
+    assert ef < max(RES_TOL, 2.0 * floor)
 
 
 @pytest.mark.parametrize("name", sorted(CASES) + sorted(LONG_CASES))
