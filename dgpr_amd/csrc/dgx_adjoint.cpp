@@ -16,14 +16,16 @@
 //   l1 = dt/6 l + dt/2 a1,  a0 = f_V(V)^T l1
 //   adjoint of V = l + a0 + a1 + a2 + a3,
 // and dJ/dmu gains sum_i l_i . df/dmu(S_(i-1)), df/dmu = (f(S; mu) - f(S; 0))/mu
-// because every viscous and BR2 term is linear in mu.  Schedules: with
-// memory to spare, the forward sweep writes every stage state and its trace
-// records (and f) into one store and the reverse sweep only applies the four
-// residual adjoints per step; otherwise checkpoints every `interval` steps
-// (default ceil(sqrt(N))), each segment recomputed once with its step-start
-// states kept, so the reverse sweep costs one extra forward run plus, per
-// step, three stage recomputations and four residual adjoints.  Both run the
-// same kernels on the same inputs, so their gradients agree bit for bit.
+// because every viscous and BR2 term is linear in mu; f(S; 0) is the
+// inviscid right-hand side, evaluated from S and its stored records.
+// Schedules: with memory to spare, the forward sweep writes every stage
+// state and its trace records (and f) into one store and the reverse sweep
+// only applies the four residual adjoints per step; otherwise checkpoints
+// every `interval` steps (default ceil(sqrt(N))), each segment replayed
+// once into a store of its steps' stages (or, when even that does not fit,
+// its step-start states recomputed and every step's stages rebuilt).  All
+// run the same kernels on the same inputs, so their gradients agree bit for
+// bit.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -270,25 +272,20 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   const size_t n = c->nstate;
   const size_t ntr = trace_doubles(c->p, c->nx, c->ny, c->nz);
 
-  DevBuf lam, lk, acur, asum, tz, f0, dpart, dsum, final_state;
+  DevBuf lam, lk, acur, asum, f0, dpart, dsum, final_state;
   AHIP(c, alloc(lam, n));
   AHIP(c, alloc(lk, n));
   AHIP(c, alloc(acur, n));
   AHIP(c, alloc(asum, n));
   AHIP(c, alloc(final_state, n));
   if (dJdmu)
-  {
-    AHIP(c, alloc(tz, ntr));
     AHIP(c, alloc(f0, n));
-  }
   const int ndot = 1024;
   AHIP(c, alloc(dpart, ndot));
   // one dot product per reverse stage for dJ/dmu, read back once at the end
   AHIP(c, alloc(dsum, dJdmu ? 4 * nsteps : 1));
   AdjScratch scr;
   AHIP(c, alloc_scratch(c, scr));
-  Phys ph0 = c->phys;
-  ph0.mu = 0.0;
 
   // Schedule.  interval == 0: keep every stage state with its trace records
   // (and f for dJ/dmu) from the forward sweep when that fits in three
@@ -364,10 +361,10 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
     {
       if (dJdmu)
       {
-        // df/dmu at S_i: (k_(i+1) - f(S_i; 0)) / mu
-        AHIP(c, trace_dispatch(c->p, c->q, mesh_args(c), ph0, Si[i], tz.p, c->status,
-                               c->stream));
-        AHIP(c, stage(c, ph0, Si[i], tz.p, nullptr, nullptr, nullptr, f0.p, 0.0));
+        // df/dmu at S_i: (k_(i+1) - f(S_i; 0)) / mu, f(S_i; 0) the inviscid
+        // right-hand side from S_i and its records
+        AHIP(c, inviscid_rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, Si[i], Ti[i], f0.p,
+                                      c->lam, c->status, c->stream));
         AHIP(c, dot_dispatch(lk.p, Ki[i], f0.p, 1.0 / c->phys.mu, n, dpart.p, ndot,
                              dsum.p + ndmu, c->stream));
         ++ndmu;

@@ -21,6 +21,17 @@ void dgx_rhs_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __
   stage_body<P, Q, TT, AFF>(mesh, ph, sa, lam, status, blockIdx.x);
 }
 
+// The inviscid right-hand side of the one-row body (INV): f(V; mu = 0) from
+// the state and its trace records, whose tractions are not read.
+template <int P, int Q, bool AFF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+void dgx_inviscid_rhs_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
+                             int* status)
+{
+  __shared__ __attribute__((aligned(16))) unsigned char lds[StageLds<P, Q>::bytes];
+  stage_body_lds<P, Q, double, AFF, true>(mesh, ph, sa, lam, status, blockIdx.x, lds);
+}
+
 // Batched form: the blocks of several independent meshes of the same (P, Q)
 // in one launch.  items[] is sorted by first block; each block finds its
 // member by a uniform binary search.
@@ -741,6 +752,36 @@ hipError_t rhs_batch_dispatch(int p, int q, const BatchItem* items_dev, int nite
 #define CALL_RB(P, Q) launch_rhs_batch<P, Q>(items_dev, nitems, nblocks, status, waves, affine, st)
   DGX_DISPATCH_PQ(p, q, CALL_RB)
 #undef CALL_RB
+}
+
+template <int P, int Q>
+static hipError_t launch_inviscid(const MeshArgs& m, const Phys& ph, const double* V,
+                                  const double* Tin, double* fout, const double* lam, int* status,
+                                  hipStream_t st)
+{
+  StageArgs sa{};
+  sa.Vin     = V;
+  sa.Tin     = Tin;
+  sa.fout    = fout;
+  sa.ca      = 1.0;
+  sa.accmode = 0;
+  const int nblk = stage_blocks(P, m.nelem);
+  if (Q == 1 && m.affine)
+    hipLaunchKernelGGL((dgx_inviscid_rhs_kernel<P, Q, Q == 1>), dim3(nblk), dim3(64), 0, st, m,
+                       ph, sa, lam, status);
+  else
+    hipLaunchKernelGGL((dgx_inviscid_rhs_kernel<P, Q, false>), dim3(nblk), dim3(64), 0, st, m, ph,
+                       sa, lam, status);
+  return hipGetLastError();
+}
+
+hipError_t inviscid_rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
+                                 const double* Tin, double* fout, const double* lam, int* status,
+                                 hipStream_t st)
+{
+#define CALL_INV(P, Q) launch_inviscid<P, Q>(m, ph, V, Tin, fout, lam, status, st)
+  DGX_DISPATCH_PQ(p, q, CALL_INV)
+#undef CALL_INV
 }
 
 int stage_blocks(int p, int nelem)

@@ -108,6 +108,11 @@ hipError_t rhs_batch_dispatch(int p, int q, const BatchItem* items_dev, int nite
                               int* status, int waves, bool affine, hipStream_t st);
 // grid blocks of one mesh in a stage launch
 int stage_blocks(int p, int nelem);
+// fout = f(V; mu = 0), the inviscid right-hand side, from V and its trace
+// records Tin (written with any mu: their tractions are not read)
+hipError_t inviscid_rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
+                                 const double* Tin, double* fout, const double* lam, int* status,
+                                 hipStream_t st);
 
 // ---- ensemble stage kernel of the one-launch layouts (dgx_ensemble_kernels.hip)
 // One launch covers the units (rows or tiles of elements) of every member of

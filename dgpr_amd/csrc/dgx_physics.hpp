@@ -207,4 +207,29 @@ __device__ __forceinline__ void volume_flux(const T* Vq, const G gr[3][4], const
   }
 }
 
+// Inviscid part of volume_flux (H = -F, the Euler flux): the values
+// volume_flux gives with mu = 0, whose viscous terms are exact zeros.
+template <typename T>
+__device__ __forceinline__ void euler_volume_flux(const T* Vq, double gamma, T H[3][5])
+{
+  T ir = rcp(Vq[0]);
+  T u[3] = {Vq[1] * ir, Vq[2] * ir, Vq[3] * ir};
+  T p  = (gamma - 1.0) * (Vq[4] - 0.5 * Vq[0] * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+  T Ht = (Vq[4] + p) * ir;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    T md    = Vq[1 + d];
+    H[d][0] = -md;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+      H[d][1 + i] = -(md * u[i]);
+      if (i == d)
+        H[d][1 + i] -= p;
+    }
+    H[d][4] = -(md * Ht);
+  }
+}
+
 }  // namespace dgx

@@ -568,8 +568,12 @@ struct StageLds
 // holds the BR2 lifting coefficient of element e's side of its face f.
 // Body of the stage kernel for block `block` of one mesh, in the LDS region
 // lds (StageLds<P, Q>::bytes, 16-byte aligned): shared by the single-mesh
-// kernel, the batched ensemble kernel and the persistent ensemble kernel.
-template <int P, int Q, typename TT, bool AFF = false>
+// kernel and the batched ensemble kernel.  INV: the inviscid right-hand
+// side only (Euler volume flux, Roe / boundary Euler face flux; no
+// gradients, tractions, BR2 or dual-consistency terms), which equals the
+// full one at mu = 0 (whose viscous terms are exact zeros) without reading
+// the tractions of the records: the adjoint's d f / d mu.
+template <int P, int Q, typename TT, bool AFF = false, bool INV = false>
 __device__ __forceinline__ void stage_body_lds(const MeshArgs& mesh, const Phys& ph,
                                                const StageArgs& sa,
                                                const double* __restrict__ lam, int* status,
@@ -630,26 +634,32 @@ __device__ __forceinline__ void stage_body_lds(const MeshArgs& mesh, const Phys&
     double J[3][3], det;
     geometry_at<Q, AFF>(qt, sX[le], sG[le], -1, 0, idx[0], idx[1], idx[2], J, Ji, det);
     wJ = pt.w[idx[0]] * pt.w[idx[1]] * pt.w[idx[2]] * det;
-    double Vq[5], gr[3][4];
+    double Vq[5];
 #pragma unroll
     for (int r = 0; r < 5; ++r)
       Vq[r] = sV[le][r][nd0];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
+    if constexpr (INV)
+      euler_volume_flux(Vq, ph.gamma, H);
+    else
     {
-      double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+      double gr[3][4];
 #pragma unroll
-      for (int m = 0; m < N; ++m)
+      for (int r = 0; r < 4; ++r)
       {
-        g0 += pt.Dg[idx[0]][m] * sV[le][r][nid<N>(m, idx[1], idx[2])];
-        g1 += pt.Dg[idx[1]][m] * sV[le][r][nid<N>(idx[0], m, idx[2])];
-        g2 += pt.Dg[idx[2]][m] * sV[le][r][nid<N>(idx[0], idx[1], m)];
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+        {
+          g0 += pt.Dg[idx[0]][m] * sV[le][r][nid<N>(m, idx[1], idx[2])];
+          g1 += pt.Dg[idx[1]][m] * sV[le][r][nid<N>(idx[0], m, idx[2])];
+          g2 += pt.Dg[idx[2]][m] * sV[le][r][nid<N>(idx[0], idx[1], m)];
+        }
+        gr[0][r] = g0;
+        gr[1][r] = g1;
+        gr[2][r] = g2;
       }
-      gr[0][r] = g0;
-      gr[1][r] = g1;
-      gr[2][r] = g2;
+      volume_flux(Vq, gr, Ji, ph.gamma, ph.mu, H);
     }
-    volume_flux(Vq, gr, Ji, ph.gamma, ph.mu, H);
   }
 
   // ---- volume divergence, one reference direction at a time ----
@@ -703,7 +713,43 @@ __device__ __forceinline__ void stage_body_lds(const MeshArgs& mesh, const Phys&
 
     FacePoint fpo;
     double hn[4], h1[4], h2[4];
-    if (fact)
+    if (INV && fact)
+    {
+      // Roe flux (interior) or the Euler flux of the boundary state; no
+      // viscous face terms: the contravariant dual-consistency flux is zero
+      const int en = sNb[l][f];
+      const TT* To = Tin + ((size_t)ee * 6 + f) * 8 * NF + fpt;
+      double UL[5], Ub[5];
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+        UL[r] = To[r * NF];
+      const TT* Tn = (en >= 0) ? Tin + ((size_t)en * 6 + (f ^ 1)) * 8 * NF + fpt
+                               : Tin + (size_t)mesh.nelem * 6 * 8 * NF +
+                                     (size_t)(-1 - en) * 5 * NF + fpt;
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+        Ub[r] = Tn[r * NF];
+      const int gi0 = (nd == 0) ? 0 : a;
+      const int gi1 = (nd == 1) ? 0 : ((nd == 0) ? a : b);
+      const int gi2 = (nd == 2) ? 0 : b;
+      double JiL[3][3], nv[3];
+      const double fJ = face_geometry<Q, AFF>(qt, sX[l], sG[l], nd, hi, gi0, gi1, gi2, JiL, nv);
+      if (en >= 0)
+        roe_flux(UL, Ub, nv, ph.gamma, fpo.G);
+      else
+        euler_flux_n(Ub, ph.gamma, nv, fpo.G);
+      const double sc = fJ * pt.w[a] * pt.w[b];
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+      {
+        fpo.G[r] *= sc;
+        sR[tid * FS + r] = fpo.G[r];
+      }
+#pragma unroll
+      for (int r = 0; r < 12; ++r)
+        sR[tid * FS + 5 + r] = 0.0;
+    }
+    else if (fact)
     {
       const int en = sNb[l][f];
       const TT* To = Tin + ((size_t)ee * 6 + f) * 8 * NF + fpt;
