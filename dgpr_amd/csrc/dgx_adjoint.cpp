@@ -457,13 +457,26 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   }
   else
   {
+    // the last segment goes straight into the segment store (the reverse
+    // sweep starts with it and need not replay it)
+    const uint64_t s_last = (nchk - 1) * K;
     for (uint64_t s = 0; s < nsteps; ++s)
     {
       if (s % K == 0)
         AHIP(c, copy_state(c, chk[s / K].p, c->U));
+      if (seg_store && s == s_last)
+      {
+        dgx_status st = forward_store(c->U, nsteps - s_last, final_state.p);
+        if (st != DGX_OK)
+          return st;
+        AHIP(c, copy_state(c, c->U, final_state.p));
+        c->tu_valid = false;
+        break;
+      }
       AHIP(c, step_ctx(c, dt));
     }
-    AHIP(c, copy_state(c, final_state.p, c->U));
+    if (!seg_store)
+      AHIP(c, copy_state(c, final_state.p, c->U));
   }
 
   // objective and its gradient at the final state
@@ -482,12 +495,13 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
   }
   else if (seg_store)
   {
-    // per segment: replay its steps once into the store, reverse them
+    // per segment: replay its steps once into the store (the last one is
+    // there from the forward sweep), reverse them
     for (uint64_t sg = nchk; sg-- > 0;)
     {
       const uint64_t s0 = sg * K, s1 = (s0 + K < nsteps) ? s0 + K : nsteps;
       // the segment's end state is not needed again: acur is free here
-      dgx_status st = forward_store(chk[sg].p, s1 - s0, acur.p);
+      dgx_status st = (sg + 1 < nchk) ? forward_store(chk[sg].p, s1 - s0, acur.p) : DGX_OK;
       if (st == DGX_OK)
         st = reverse_store(s1 - s0);
       if (st != DGX_OK)

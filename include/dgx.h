@@ -297,7 +297,8 @@ dgx_status dgx_residual_vjp_device_noalloc(dgx_ctx* ctx, const double* w_dev, do
  * may be NULL) and dJ/dmu (may be NULL), by the discrete adjoint.
  * interval >= 1: a checkpoint every `interval` steps; in the reverse sweep
  * each segment is replayed once into a store of `interval` steps' stage
- * states with their trace records (and f, for dJ/dmu), or, when that store
+ * states with their trace records (and f, for dJ/dmu; the last segment is
+ * written there by the forward sweep itself), or, when that store
  * does not fit, its step-start states are recomputed and each step's stages
  * rebuilt.  interval = 0: every stage state with its records is kept from
  * the forward sweep when nsteps * 4 * (state + records [+ state]) fits in
