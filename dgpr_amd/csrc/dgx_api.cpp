@@ -82,6 +82,7 @@ MeshArgs dgx::mesh_args(const dgx_ctx* c)
   for (int i = 0; i < 6; ++i)
     m.bounds[i] = c->bounds[i];
   m.affine = c->affine;
+  m.lamf   = c->lamf;
   return m;
 }
 
@@ -280,7 +281,8 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
   c->stage_waves = c->stage_tile ? (c->affine ? 3 : 2) : default_stage_waves(c->p, c->q);
   c->prec = DGX_PREC_FP64;
   c->bytes = 5 * sb + 3 * tb +
-             (nn + nlam + (size_t)c->npartial + sum_scratch(c->npartial)) * sizeof(double) + 16;
+             (nn + nlam + (c->affine ? (size_t)c->nelem * 6 : 0) + (size_t)c->npartial +
+              sum_scratch(c->npartial)) * sizeof(double) + 16;
   if ((e = hipMemcpyAsync(c->nodes, mesh->nodes, nn * sizeof(double), hipMemcpyHostToDevice,
                           c->stream)) != hipSuccess)
     return bail("hipMemcpy(nodes)", e);
@@ -288,6 +290,13 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
     return bail("hipMemset(U)", e);
   if ((e = lift_coef_dispatch(c->p, c->q, mesh_args(c), c->lam, c->stream)) != hipSuccess)
     return bail("lifting coefficients", e);
+  if (c->affine)
+  {
+    if ((e = hipMalloc(&c->lamf, (size_t)c->nelem * 6 * sizeof(double))) != hipSuccess)
+      return bail("hipMalloc(lamf)", e);
+    if ((e = lift_face_dispatch(c->p, c->lam, c->nelem, c->lamf, c->stream)) != hipSuccess)
+      return bail("face lifting coefficients", e);
+  }
   if ((e = hipMemsetAsync(c->status, 0, sizeof(int), c->stream)) != hipSuccess)
     return bail("hipMemset(status)", e);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
@@ -302,8 +311,8 @@ dgx_status dgx_destroy(dgx_ctx* c)
     return DGX_OK;
   if (c->stream)
     (void)hipStreamSynchronize(c->stream);
-  double* bufs[13] = {c->U,   c->Ua,  c->Ub,  c->acc, c->fs,  c->nodes, c->partial,
-                      c->sum, c->lam, c->TU,  c->TUa, c->TUb, c->mean};
+  double* bufs[14] = {c->U,   c->Ua,  c->Ub,  c->acc, c->fs,  c->nodes, c->partial,
+                      c->sum, c->lam, c->lamf, c->TU, c->TUa, c->TUb, c->mean};
   for (double* b : bufs)
     if (b)
       (void)hipFree(b);

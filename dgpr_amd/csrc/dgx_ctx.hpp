@@ -28,6 +28,7 @@ struct dgx_ctx
   double* partial;  // per-block sums of R^2
   double* sum;      // reduced sum
   double* lam;      // BR2 lifting coefficients [e][6][(p+1)^2]
+  double* lamf;     // affine meshes: one coefficient per element face [e][6]
   double* TU;       // face trace records of U, Ua, Ub [e][6][8][(p+1)^2]
   double* TUa;
   double* TUb;

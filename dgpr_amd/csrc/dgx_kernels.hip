@@ -798,6 +798,22 @@ hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const
 #undef CALL_TR
 }
 
+__global__ __launch_bounds__(256) void dgx_lift_face_kernel(const double* __restrict__ lam, int nf,
+                                                           size_t n, double* __restrict__ lamf)
+{
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n)
+    lamf[i] = lam[i * nf];
+}
+
+hipError_t lift_face_dispatch(int p, const double* lam, int nelem, double* lamf, hipStream_t st)
+{
+  const size_t n = (size_t)nelem * 6;
+  hipLaunchKernelGGL(dgx_lift_face_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lam,
+                     (p + 1) * (p + 1), n, lamf);
+  return hipGetLastError();
+}
+
 hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipStream_t st)
 {
 #define CALL_LC(P, Q) launch_lift_coef<P, Q>(m, lam, st)

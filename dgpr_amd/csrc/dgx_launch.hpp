@@ -51,6 +51,10 @@ struct MeshArgs
   int nelem;
   int bounds[6];
   int affine;  // every element is a q = 1 parallelepiped (constant Jacobian)
+  // affine meshes: the BR2 lifting coefficient of each element face [e][6],
+  // constant over a parallelepiped's face (fJ / det J times a table sum);
+  // null otherwise
+  const double* lamf;
 };
 
 // Epilogue of one right-hand-side evaluation:
@@ -134,6 +138,8 @@ hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const
 size_t trace_doubles(int p, int nx, int ny, int nz);
 // lam[e][f][(p+1)^2]: BR2 lifting coefficient of each element side of each face
 hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipStream_t st);
+// lamf[e][f] = lam[e][f][0] (affine meshes: the face's constant value)
+hipError_t lift_face_dispatch(int p, const double* lam, int nelem, double* lamf, hipStream_t st);
 hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg, double* f,
                               double* R, double* partial, hipStream_t st);
 hipError_t basis_dispatch(int p, const double* in, double* out, int nelem, int mode,

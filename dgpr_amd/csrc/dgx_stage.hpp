@@ -767,7 +767,9 @@ __device__ __forceinline__ void stage_body_lds(const MeshArgs& mesh, const Phys&
       const int gi2 = (nd == 2) ? 0 : b;
       double JiL[3][3], nv[3];
       const double fJ = face_geometry<Q, AFF>(qt, sX[l], sG[l], nd, hi, gi0, gi1, gi2, JiL, nv);
-      const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
+      // affine meshes: the face's constant coefficient
+      const double lamL =
+          AFF ? mesh.lamf[(size_t)ee * 6 + f] : lam[((size_t)ee * 6 + f) * NF + fpt];
 
       // neighbour trace (its traction was taken with its own outward normal
       // -n), or the boundary state stored with the traces
@@ -782,7 +784,8 @@ __device__ __forceinline__ void stage_body_lds(const MeshArgs& mesh, const Phys&
 #pragma unroll
         for (int i = 0; i < 3; ++i)
           vgR[i] = -Tn[(5 + i) * NF];
-        lamR = lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
+        lamR = AFF ? mesh.lamf[(size_t)en * 6 + (f ^ 1)]
+                   : lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
       }
       else
       {

@@ -117,7 +117,10 @@ __device__ __forceinline__ void tile_face_subpass(
     gi[T2] = b;
     double JiL[3][3], nv[3];
     const double fJ = face_geometry<Q, AFF>(qt, sX[l], sG[l], D, hi, gi[0], gi[1], gi[2], JiL, nv);
-    const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
+    // affine meshes: the face's constant coefficient (48 B per element
+    // instead of 6 (p+1)^2 doubles, read for the own and the neighbour side)
+    const double lamL =
+        AFF ? mesh.lamf[(size_t)ee * 6 + f] : lam[((size_t)ee * 6 + f) * NF + fpt];
     double Ub[5], vgR[3] = {0.0, 0.0, 0.0};
     double lamR = 0.0;
     if (en >= 0)
@@ -129,7 +132,8 @@ __device__ __forceinline__ void tile_face_subpass(
 #pragma unroll
       for (int i = 0; i < 3; ++i)
         vgR[i] = -Tn[(5 + i) * NF];
-      lamR = lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
+      lamR = AFF ? mesh.lamf[(size_t)en * 6 + (f ^ 1)]
+                 : lam[((size_t)en * 6 + (f ^ 1)) * NF + fpt];
     }
     else
     {
