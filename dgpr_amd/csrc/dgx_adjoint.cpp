@@ -420,8 +420,8 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
           sa.Tout = (s + 1 < ns) ? Tp(s + 1, 0) : nullptr;
         }
         sa.fout    = dJdmu ? Kp(s, i) : nullptr;
-        sa.ca      = 1.0;
-        sa.cb      = 0.0;
+        sa.ca      = (i == 0) ? 0.0 : 1.0;  // stage 1: its base is its input (in LDS)
+        sa.cb      = (i == 0) ? 1.0 : 0.0;
         sa.cf      = cstage[i];
         sa.accmode = accmode[i];
         sa.trace32 = 0;

@@ -515,15 +515,17 @@ int dgx::stage_plan(const dgx_ctx* c, int scheme, double dt, StageArgs sa[4])
     a.trace32 = t32;
     a.ctype   = ct;
   };
+  // the first stage's base is its own input: cb = 1 takes it from the LDS
+  // copy the kernel already holds instead of reading U from HBM again
   if (scheme == DGX_RK4)
   {
-    set(sa[0], c->U, c->TU, c->U, c->acc, c->Ua, c->TUa, 1.0, 0.0, 0.5 * dt, 1);  // U1, acc = f1
+    set(sa[0], c->U, c->TU, c->U, c->acc, c->Ua, c->TUa, 0.0, 1.0, 0.5 * dt, 1);  // U1, acc = f1
     set(sa[1], c->Ua, c->TUa, c->U, c->acc, c->Ub, c->TUb, 1.0, 0.0, 0.5 * dt, 2);  // U2, acc += 2 f2
     set(sa[2], c->Ub, c->TUb, c->U, c->acc, c->Ua, c->TUa, 1.0, 0.0, dt, 2);  // U3, acc += 2 f3
     set(sa[3], c->Ua, c->TUa, c->U, c->acc, c->U, c->TU, 1.0, 0.0, dt, 3);  // U += dt/6 (acc + f4)
     return 4;
   }
-  set(sa[0], c->U, c->TU, c->U, nullptr, c->Ua, c->TUa, 1.0, 0.0, dt, 0);  // U1 = U + dt f
+  set(sa[0], c->U, c->TU, c->U, nullptr, c->Ua, c->TUa, 0.0, 1.0, dt, 0);  // U1 = U + dt f
   set(sa[1], c->Ua, c->TUa, c->U, nullptr, c->Ub, c->TUb, 0.75, 0.25, 0.25 * dt, 0);
   set(sa[2], c->Ub, c->TUb, c->U, nullptr, c->U, c->TU, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt,
       0);
