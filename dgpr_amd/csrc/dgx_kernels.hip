@@ -1,7 +1,7 @@
 // dgx_kernels.hip -- gfx950 kernels of the FP64 DG residual + RK stage, the
 // basis conversions, residual output, derived outputs and the discrete
 // adjoint, with their host launch helpers.  The device code shared with the
-// persistent ensemble kernel lives in dgx_stage.hpp (method described there).
+// ensemble stage kernel lives in dgx_stage.hpp (method described there).
 #include "dgx_stage.hpp"
 #include "dgx_dual.hpp"
 

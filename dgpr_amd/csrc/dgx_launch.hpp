@@ -85,8 +85,8 @@ struct StageArgs
 
 // One member of a batched stage launch: its mesh, physics, stage arguments
 // and lifting coefficients, the first grid block (or work unit) of its
-// elements, and its order (read by the persistent kernel, which holds the
-// bodies of every order).
+// elements, and its order (read by the ensemble stage kernel, which can hold
+// the bodies of every order).
 struct BatchItem
 {
   MeshArgs mesh;

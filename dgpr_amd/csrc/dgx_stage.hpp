@@ -541,7 +541,7 @@ __global__ __launch_bounds__(64) void dgx_trace_kernel(MeshArgs mesh, Phys ph,
 }
 
 // LDS of one stage-body block, carved from a byte buffer so that kernels
-// holding the bodies of several orders (the persistent ensemble kernel) give
+// holding the bodies of several orders (the ensemble stage kernel) give
 // them one shared region: state [EPW][5][NB], own geometry nodes
 // [EPW][NG*3], affine-element records [EPW][GREC], the exchange region
 // (volume flux of one direction, then one face pass), the 1D tables and the
@@ -568,7 +568,8 @@ struct StageLds
 // holds the BR2 lifting coefficient of element e's side of its face f.
 // Body of the stage kernel for block `block` of one mesh, in the LDS region
 // lds (StageLds<P, Q>::bytes, 16-byte aligned): shared by the single-mesh
-// kernel and the batched ensemble kernel.  INV: the inviscid right-hand
+// kernel and the ensemble kernels (batched, and the one-row units of the
+// one-launch layouts).  INV: the inviscid right-hand
 // side only (Euler volume flux, Roe / boundary Euler face flux; no
 // gradients, tractions, BR2 or dual-consistency terms), which equals the
 // full one at mu = 0 (whose viscous terms are exact zeros) without reading

@@ -13,7 +13,8 @@ hold at any size:
   no-slip wall's boundary state carries no mass flux, and the residual sums
   to zero against the constant basis function, solve.cpp:484-794);
 * config 3 (the 64-case ensemble): all 64 members built, two steps of the
-  persistent launch replayed from the hipGraph equal each member's
+  default layout (one launch per stage for all orders) replayed from the
+  hipGraph equal each member's
   single-case steps to rounding (1e-13 relative: the same arithmetic in a
   differently compiled kernel);
 * config 4 (32^3 p = 2 adjoint): the residual dot-product test and the
@@ -104,7 +105,7 @@ def test_config3_full_ensemble_matches_single_cases():
         got = [m.get_state() for m in en.members]
     for i, c in enumerate(cs):
         with dgx.Solver(c.mesh, c.params) as s:
-            s.tile = 0  # the persistent kernel's units before tuning: one row
+            s.tile = 0  # the ensemble kernel's units before tuning: one row
             s.set_state(c.U0)
             s.step(c.dt, "RK4", 2, norms=False)
             ref = s.get_state()
