@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "libdgx.so")
 DRIVER = os.path.join(HERE, "dgx_drive")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["dgx_kernels.hip", "dgx_ensemble_kernels.hip", "dgx_api.cpp", "dgx_ensemble.cpp", "dgx_adjoint.cpp",
+SOURCES = ["dgx_kernels.hip", "dgx_ensemble_kernels.hip", "dgx_general.hip", "dgx_api.cpp", "dgx_ensemble.cpp", "dgx_adjoint.cpp",
            "dgx_tables.cpp", "dgx_host.cpp"]
 ARCH = os.environ.get("DGX_ARCH", "gfx950")
 

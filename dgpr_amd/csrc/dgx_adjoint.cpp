@@ -182,6 +182,8 @@ dgx_status dgx_kinetic_energy(dgx_ctx* c, double* J)
 {
   if (!c || !J)
     return DGX_ERR_INVALID;
+  if (general_order(c->p, c->q))
+    return afail(c, DGX_ERR_UNSUPPORTED, "the adjoint covers 1 <= p, q <= 3");
   DevBuf g;
   AHIP(c, alloc(g, c->nstate));
   AHIP(c, kinetic_energy_dispatch(c->p, c->q, mesh_args(c), c->U, g.p, c->partial, c->stream));
@@ -195,6 +197,8 @@ dgx_status dgx_residual_vjp(dgx_ctx* c, const double* w_host, double* g_host)
 {
   if (!c || !w_host || !g_host)
     return DGX_ERR_INVALID;
+  if (general_order(c->p, c->q))
+    return afail(c, DGX_ERR_UNSUPPORTED, "the adjoint covers 1 <= p, q <= 3");
   if (c->prec != DGX_PREC_FP64)
     return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs DGX_PREC_FP64");
   AdjScratch s;
@@ -220,6 +224,8 @@ dgx_status dgx_residual_vjp_device_noalloc(dgx_ctx* c, const double* w_dev, doub
 {
   if (!c || !w_dev || !g_dev)
     return DGX_ERR_INVALID;
+  if (general_order(c->p, c->q))
+    return afail(c, DGX_ERR_UNSUPPORTED, "the adjoint covers 1 <= p, q <= 3");
   if (c->prec != DGX_PREC_FP64)
     return afail(c, DGX_ERR_UNSUPPORTED, "the residual adjoint needs DGX_PREC_FP64");
   const size_t n = c->nstate;
@@ -265,6 +271,8 @@ dgx_status dgx_adjoint_rk4(dgx_ctx* c, double dt, uint64_t nsteps, uint64_t inte
 {
   if (!c || nsteps == 0 || !(dt > 0.0))
     return DGX_ERR_INVALID;
+  if (general_order(c->p, c->q))
+    return afail(c, DGX_ERR_UNSUPPORTED, "the adjoint covers 1 <= p, q <= 3");
   if (c->prec != DGX_PREC_FP64)
     return afail(c, DGX_ERR_UNSUPPORTED, "the discrete adjoint needs DGX_PREC_FP64");
   if (dJdmu && !(c->phys.mu > 0.0))

@@ -138,15 +138,19 @@ static dgx_status ensure_tables(int device)
       return DGX_OK;
   static PTab pt[3];
   static QTab qt[9];
+  static GenTab gt[72];
   static bool built = false;
   if (!built)
   {
     host::build_tables(pt, qt);
+    host::build_gen_tables(gt);
     built = true;
   }
   hipError_t e = upload_tables(pt, qt);
   if (e == hipSuccess)
     e = upload_tables_ensemble(pt, qt);
+  if (e == hipSuccess)
+    e = upload_gen_tables(gt);
   if (e != hipSuccess)
     return fail(nullptr, DGX_ERR_HIP, "table upload failed: %s", hipGetErrorString(e));
   g_tab_devices.push_back(device);
@@ -189,11 +193,12 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
   if (!mesh || !params || !out || !mesh->nodes)
     return fail(nullptr, DGX_ERR_INVALID, "null argument");
   *out = nullptr;
-  if (mesh->p < 1 || mesh->p > 3)
-    return fail(nullptr, DGX_ERR_UNSUPPORTED, "solution order p=%llu outside 1..3",
+  // the reference's range (MAX_ORDER, constants.cpp:21; geometry order >= 1)
+  if (mesh->p > 8)
+    return fail(nullptr, DGX_ERR_UNSUPPORTED, "solution order p=%llu outside 0..8",
                 (unsigned long long)mesh->p);
-  if (mesh->q < 1 || mesh->q > 3)
-    return fail(nullptr, DGX_ERR_UNSUPPORTED, "geometry order q=%llu outside 1..3",
+  if (mesh->q < 1 || mesh->q > 8)
+    return fail(nullptr, DGX_ERR_UNSUPPORTED, "geometry order q=%llu outside 1..8",
                 (unsigned long long)mesh->q);
   if (mesh->nx == 0 || mesh->ny == 0 || mesh->nz == 0)
     return fail(nullptr, DGX_ERR_INVALID, "empty mesh");
@@ -241,7 +246,7 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
   c->params = *params;
   derive_phys(*params, c->phys);
   size_t nn = (size_t)c->nelem * (c->q + 1) * (c->q + 1) * (c->q + 1) * 3;
-  c->npartial = resid_out_blocks(c->p, c->nelem);
+  c->npartial = resid_out_blocks(c->p, c->q, c->nelem);
 
   auto bail = [&](const char* what, hipError_t e) {
     snprintf(g_create_err, sizeof(g_create_err), "%s failed: %s", what, hipGetErrorString(e));
@@ -276,9 +281,10 @@ dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params, dgx_ctx** 
   c->tu_valid = false;
   // the tiled kernel at p = 3, q = 1 (measured 14 % faster than the one-row
   // kernel on the 64^3 cube at 3 waves per SIMD, 7 % on the curved channel
-  // at 2, where 3 spills); the one-row kernel elsewhere
-  c->stage_tile  = (c->p == 3 && c->q == 1) ? 1 : 0;
-  c->stage_waves = c->stage_tile ? (c->affine ? 3 : 2) : default_stage_waves(c->p, c->q);
+  // at 2, where 3 spills); the one-row kernel at the other orders of
+  // 1..3; the general-order kernel (tile 2) outside them
+  c->stage_tile  = general_order(c->p, c->q) ? 2 : ((c->p == 3 && c->q == 1) ? 1 : 0);
+  c->stage_waves = c->stage_tile == 1 ? (c->affine ? 3 : 2) : default_stage_waves(c->p, c->q);
   c->prec = DGX_PREC_FP64;
   c->bytes = 5 * sb + 3 * tb +
              (nn + nlam + (c->affine ? (size_t)c->nelem * 6 : 0) + (size_t)c->npartial +
@@ -559,7 +565,7 @@ static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm,
       if ((e = norm_partials_dispatch(c->p, c->q, mesh_args(c), f1, c->partial, c->stream)) !=
           hipSuccess)
         return e;
-      const int nb = norm_blocks(c->p, c->nelem);
+      const int nb = norm_blocks(c->p, c->q, c->nelem);
       e = norm_dev ? norm_dispatch(c->partial, nb, 1.0 / (double)c->nstate, norm_dev, c->stream)
                    : sum_dispatch(c->partial, nb, c->sum, c->stream);
       if (e != hipSuccess)
@@ -661,10 +667,11 @@ dgx_status dgx_get_tuning(const dgx_ctx* c, int* stage_waves)
 
 dgx_status dgx_set_tile(dgx_ctx* c, int tile)
 {
-  if (!c || tile < 0 || tile > 1)
+  if (!c || tile < 0 || tile > 2)
     return DGX_ERR_INVALID;
-  if (!tile_valid(c->p, c->q, tile, tile ? 2 : 1))
-    return fail(c, DGX_ERR_UNSUPPORTED, "no tiled stage kernel for p = %d, q = %d", c->p, c->q);
+  if (!tile_valid(c->p, c->q, tile, 2))
+    return fail(c, DGX_ERR_UNSUPPORTED, "no stage kernel of tile %d for p = %d, q = %d", tile,
+                c->p, c->q);
   c->stage_tile = tile;
   if (!tile_valid(c->p, c->q, tile, c->stage_waves))
     c->stage_waves = 2;  // the tiled kernels are compiled for 2 and 3 waves per SIMD

@@ -238,6 +238,12 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
       return st;
     }
     en->ctx.push_back(c);
+    if (general_order(c->p, c->q))
+    {
+      efail(nullptr, DGX_ERR_UNSUPPORTED, "member %d: ensembles cover 1 <= p, q <= 3", i);
+      dgx_ensemble_destroy(en);
+      return DGX_ERR_UNSUPPORTED;
+    }
   }
   en->device = en->ctx[0]->device;
   en->layout = 1;

@@ -42,11 +42,13 @@ inline double lag(int i, int p, double x)
 }
 
 // global position of the equispaced order-p node bf of element e
-// (ref2glo of lagrange_node3d, geometry_common.cpp:235-249)
+// (ref2glo of lagrange_node3d, geometry_common.cpp:235-249; the single p = 0
+// node sits at the reference origin, basis.cpp:64-70)
 void node_position(const dgx_mesh* m, uint64_t e, int bf, double* x)
 {
   int p = (int)m->p, q = (int)m->q, n = p + 1, nq = q + 1;
-  double r[3] = {(double)(bf % n) / p, (double)((bf / n) % n) / p, (double)(bf / (n * n)) / p};
+  const double dp = p ? (double)p : 1.0;  // p = 0: the only index is 0
+  double r[3] = {(double)(bf % n) / dp, (double)((bf / n) % n) / dp, (double)(bf / (n * n)) / dp};
   x[0] = x[1] = x[2] = 0.0;
   const double* X = m->nodes + e * (uint64_t)nq * nq * nq * 3;
   for (int cz = 0; cz < nq; ++cz)
@@ -63,7 +65,7 @@ void node_position(const dgx_mesh* m, uint64_t e, int bf, double* x)
 
 bool mesh_ok(const dgx_mesh* m)
 {
-  return m && m->nodes && m->p >= 1 && m->p <= 8 && m->q >= 1 && m->q <= 8 && m->nx > 0 &&
+  return m && m->nodes && m->p <= 8 && m->q >= 1 && m->q <= 8 && m->nx > 0 &&
          m->ny > 0 && m->nz > 0;
 }
 

@@ -4,6 +4,7 @@
 // ensemble stage kernel lives in dgx_stage.hpp (method described there).
 #include "dgx_stage.hpp"
 #include "dgx_dual.hpp"
+#include "dgx_dd.hpp"
 
 #include <type_traits>
 
@@ -135,42 +136,6 @@ __device__ __forceinline__ void apply_dir(double (*s)[5][NB], const double (*M)[
       s[le][r][node] = out[r];
   }
   __syncthreads();
-}
-
-// Double-double arithmetic (hi + lo) for the state basis conversions: the
-// state itself is known to 1 ulp, and the residual's sensitivity to the state
-// (the stiff acoustic part of the operator) would amplify a conversion error
-// of a few ulps, so V = B3 U is formed with error-free transformations
-// (Knuth two-sum, FMA two-product) and rounded once.
-struct DD
-{
-  double hi, lo;
-};
-
-__device__ __forceinline__ DD dd_two_sum(double a, double b)
-{
-  double s  = a + b;
-  double bb = s - a;
-  double e  = (a - (s - bb)) + (b - bb);
-  return {s, e};
-}
-
-__device__ __forceinline__ DD dd_fast(double a, double b)
-{
-  double s = a + b;
-  return {s, b - (s - a)};
-}
-
-// (x.hi + x.lo) * (c + clo) + acc, all in double-double
-__device__ __forceinline__ DD dd_fma(DD x, double c, double clo, DD acc)
-{
-  double p = x.hi * c;
-  double e = fma(x.hi, c, -p);
-  e = fma(x.hi, clo, e);
-  e = fma(x.lo, c, e);
-  DD s = dd_two_sum(acc.hi, p);
-  s.lo += acc.lo + e;
-  return dd_fast(s.hi, s.lo);
 }
 
 // mode 0: V = B3 U      (reference basis -> Gauss-nodal)
@@ -324,7 +289,7 @@ __global__ __launch_bounds__(64) void dgx_resid_out_kernel(MeshArgs mesh, const 
 // dgx_resid_out_kernel), for the per-step residual norm: 4 x EPW elements
 // per 256-lane workgroup, the 1D tables copied to LDS (lane-dependent table
 // reads would otherwise be vector loads from constant memory), one butterfly
-// per wavefront.  partial[blockIdx.x] of norm_blocks(p, nelem) workgroups.
+// per wavefront.  partial[blockIdx.x] of norm_blocks(p, q, nelem) workgroups.
 template <int P, int Q>
 __global__ __launch_bounds__(256) void dgx_norm_kernel(MeshArgs mesh, const double* __restrict__ fg,
                                                        double* __restrict__ partial)
@@ -707,8 +672,11 @@ size_t trace_doubles(int p, int nx, int ny, int nz)
   return (size_t)nx * ny * nz * 6 * 8 * nf + boundary_faces(nx, ny, nz) * 5 * nf;
 }
 
-int resid_out_blocks(int p, int nelem)
+// general orders: one workgroup (and one partial) per element
+int resid_out_blocks(int p, int q, int nelem)
 {
+  if (general_order(p, q))
+    return nelem;
   int nb = (p + 1) * (p + 1) * (p + 1);
   int epw = 64 / nb;
   return (nelem + epw - 1) / epw;
@@ -730,10 +698,14 @@ int resid_out_blocks(int p, int nelem)
   return hipErrorInvalidValue;
 
 // the tiled kernels are compiled for q = 1 (with higher-order geometry the
-// extra rows of accumulators spill)
+// extra rows of accumulators spill); tile 2, the general-order kernel, runs
+// every order (the only one outside 1 <= p, q <= 3; no occupancy targets)
 bool tile_valid(int p, int q, int tile, int waves)
 {
-  return tile == 0 || (tile == 1 && q == 1 && tile_rows(p) > 0 && (waves == 2 || waves == 3));
+  if (general_order(p, q))
+    return tile == 2;
+  return tile == 0 || tile == 2 ||
+         (tile == 1 && q == 1 && tile_rows(p) > 0 && (waves == 2 || waves == 3));
 }
 
 hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
@@ -741,6 +713,8 @@ hipError_t rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const S
 {
   if (!tile_valid(p, q, tile, waves))
     return hipErrorInvalidValue;
+  if (tile == 2)
+    return gen_rhs_dispatch(p, q, m, ph, sa, lam, status, st);
 #define CALL_RHS(P, Q) launch_rhs<P, Q>(m, ph, sa, lam, status, waves, tile, st)
   DGX_DISPATCH_PQ(p, q, CALL_RHS)
 #undef CALL_RHS
@@ -793,6 +767,8 @@ int stage_blocks(int p, int nelem)
 hipError_t trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
                           double* Tout, int* status, hipStream_t st, bool trace32)
 {
+  if (general_order(p, q))
+    return gen_trace_dispatch(p, q, m, ph, V, Tout, status, st, trace32);
 #define CALL_TR(P, Q) launch_trace<P, Q>(m, ph, V, Tout, status, st, trace32)
   DGX_DISPATCH_PQ(p, q, CALL_TR)
 #undef CALL_TR
@@ -816,13 +792,17 @@ hipError_t lift_face_dispatch(int p, const double* lam, int nelem, double* lamf,
 
 hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipStream_t st)
 {
+  if (general_order(p, q))
+    return gen_lift_coef_dispatch(p, q, m, lam, st);
 #define CALL_LC(P, Q) launch_lift_coef<P, Q>(m, lam, st)
   DGX_DISPATCH_PQ(p, q, CALL_LC)
 #undef CALL_LC
 }
 
-int norm_blocks(int p, int nelem)
+int norm_blocks(int p, int q, int nelem)
 {
+  if (general_order(p, q))
+    return nelem;
   const int epb = 4 * (64 / ((p + 1) * (p + 1) * (p + 1)));
   return (nelem + epb - 1) / epb;
 }
@@ -831,7 +811,7 @@ template <int P, int Q>
 static hipError_t launch_norm(const MeshArgs& m, const double* fg, double* partial,
                               hipStream_t st)
 {
-  hipLaunchKernelGGL((dgx_norm_kernel<P, Q>), dim3(norm_blocks(P, m.nelem)), dim3(256), 0, st, m,
+  hipLaunchKernelGGL((dgx_norm_kernel<P, Q>), dim3(norm_blocks(P, Q, m.nelem)), dim3(256), 0, st, m,
                      fg, partial);
   return hipGetLastError();
 }
@@ -839,6 +819,8 @@ static hipError_t launch_norm(const MeshArgs& m, const double* fg, double* parti
 hipError_t norm_partials_dispatch(int p, int q, const MeshArgs& m, const double* fg,
                                   double* partial, hipStream_t st)
 {
+  if (general_order(p, q))
+    return gen_resid_out_dispatch(p, q, m, fg, nullptr, nullptr, partial, st);
 #define CALL_NM(P, Q) launch_norm<P, Q>(m, fg, partial, st)
   DGX_DISPATCH_PQ(p, q, CALL_NM)
 #undef CALL_NM
@@ -847,6 +829,8 @@ hipError_t norm_partials_dispatch(int p, int q, const MeshArgs& m, const double*
 hipError_t resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg, double* f,
                               double* R, double* partial, hipStream_t st)
 {
+  if (general_order(p, q))
+    return gen_resid_out_dispatch(p, q, m, fg, f, R, partial, st);
 #define CALL_RO(P, Q) launch_resid_out<P, Q>(m, fg, f, R, partial, st)
   DGX_DISPATCH_PQ(p, q, CALL_RO)
 #undef CALL_RO
@@ -861,7 +845,7 @@ hipError_t basis_dispatch(int p, const double* in, double* out, int nelem, int m
     case 2: return launch_basis<2>(in, out, nelem, mode, st);
     case 3: return launch_basis<3>(in, out, nelem, mode, st);
   }
-  return hipErrorInvalidValue;
+  return gen_basis_dispatch(p, in, out, nelem, mode, st);
 }
 
 hipError_t sample_dispatch(int p, const double* V, int nelem, int Ns, int var, double gamma,
@@ -883,7 +867,7 @@ hipError_t sample_dispatch(int p, const double* V, int nelem, int Ns, int var, d
       hipLaunchKernelGGL(dgx_sample_kernel<3>, dim3(nblk), dim3(256), 0, st, V, nelem, Ns, var,
                          gamma, out);
       break;
-    default: return hipErrorInvalidValue;
+    default: return gen_sample_dispatch(p, V, nelem, Ns, var, gamma, out, st);
   }
   return hipGetLastError();
 }

@@ -34,6 +34,22 @@ struct QTab
   double Be[2][MAXN], Bde[2][MAXN];
 };
 
+// Tables of one (p, q) pair for the general-order kernels (dgx_general.hip),
+// which cover every order the reference accepts: 0 <= p <= 8, 1 <= q <= 8.
+// Same objects as PTab and QTab (Bq[i][c], Bqd[i][c]: geometry-order
+// equispaced Lagrange value / derivative of node c at Gauss point i; Be, Bde
+// at the two ends of [0,1]).
+constexpr int GMAXN = 9;  // (p+1) <= 9 and (q+1) <= 9
+struct GenTab
+{
+  double g[GMAXN], w[GMAXN];
+  double B[GMAXN][GMAXN], Binv[GMAXN][GMAXN], Dg[GMAXN][GMAXN];
+  double L[2][GMAXN], Ld[2][GMAXN];
+  double Blo[GMAXN][GMAXN], Binvlo[GMAXN][GMAXN];
+  double Bq[GMAXN][GMAXN], Bqd[GMAXN][GMAXN];
+  double Be[2][GMAXN], Bde[2][GMAXN];
+};
+
 // Physical constants of one evaluation (parameters + derived boundary data).
 struct Phys
 {
@@ -181,16 +197,41 @@ hipError_t mean_dispatch(const double* U, double* mean, size_t n, double old_ran
                          double new_range, hipStream_t st);
 hipError_t upload_tables(const PTab* pt, const QTab* qt);
 hipError_t copy_dispatch(const void* src, void* dst, size_t bytes, hipStream_t st);
-int resid_out_blocks(int p, int nelem);
+int resid_out_blocks(int p, int q, int nelem);
 // per-workgroup sums of R^2 (R the modal residual of f_g) into
-// partial[0 .. norm_blocks(p, nelem)), for the per-step residual norm
-int norm_blocks(int p, int nelem);
+// partial[0 .. norm_blocks(p, q, nelem)), for the per-step residual norm
+int norm_blocks(int p, int q, int nelem);
 hipError_t norm_partials_dispatch(int p, int q, const MeshArgs& m, const double* fg,
                                   double* partial, hipStream_t st);
+
+// ---- general-order kernels (dgx_general.hip) ----
+// True when (p, q) is served by the general-order kernels only: p = 0,
+// p >= 4 or q >= 4 (the tuned kernels cover 1 <= p, q <= 3).
+inline bool general_order(int p, int q)
+{
+  return p < 1 || p > 3 || q > 3;
+}
+// tables of every (p, q), 0 <= p <= 8, 1 <= q <= 8: gt[p * 8 + q - 1]
+hipError_t upload_gen_tables(const GenTab* gt);
+// the entry points below mirror the tuned ones for the general orders
+// (one workgroup per element, one Gauss node per lane)
+hipError_t gen_rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
+                            const double* lam, int* status, hipStream_t st);
+hipError_t gen_trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,
+                              double* Tout, int* status, hipStream_t st, bool trace32);
+hipError_t gen_lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipStream_t st);
+hipError_t gen_resid_out_dispatch(int p, int q, const MeshArgs& m, const double* fg, double* f,
+                                  double* R, double* partial, hipStream_t st);
+hipError_t gen_basis_dispatch(int p, const double* in, double* out, int nelem, int mode,
+                              hipStream_t st);
+hipError_t gen_sample_dispatch(int p, const double* V, int nelem, int Ns, int var, double gamma,
+                               double* out, hipStream_t st);
 
 namespace host
 {
 void build_tables(PTab* pt, QTab* qt);
+// GenTab of every (p, q), 0 <= p <= 8, 1 <= q <= 8, at gt[p * 8 + q - 1]
+void build_gen_tables(GenTab* gt);
 }
 
 }  // namespace dgx

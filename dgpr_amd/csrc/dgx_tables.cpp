@@ -122,7 +122,7 @@ static ld lagzd(int i, int n, const ld* z, ld x)
 
 static bool invert_small(int n, const ld* A, ld* Ai)
 {
-  ld M[MAXN][2 * MAXN];
+  ld M[GMAXN][2 * GMAXN];
   for (int r = 0; r < n; ++r)
     for (int c = 0; c < 2 * n; ++c)
       M[r][c] = (c < n) ? A[r * n + c] : ((c - n == r) ? 1.0L : 0.0L);
@@ -158,6 +158,83 @@ static bool invert_small(int n, const ld* A, ld* Ai)
   return true;
 }
 
+// Solution-order tables of order p (PTab fields; T is PTab or GenTab).
+template <typename T>
+static void fill_p(int p, T& tab)
+{
+  const int n = p + 1;
+  ld g[GMAXN], w[GMAXN];
+  gauss_legendre01(n, g, w);
+  ld B[GMAXN * GMAXN], Bi[GMAXN * GMAXN];
+  for (int i = 0; i < n; ++i)
+  {
+    tab.g[i] = (double)g[i];
+    tab.w[i] = (double)w[i];
+    for (int m = 0; m < n; ++m)
+      B[i * n + m] = lag(m, p, g[i]);
+  }
+  invert_small(n, B, Bi);
+  for (int i = 0; i < n; ++i)
+    for (int m = 0; m < n; ++m)
+    {
+      tab.B[i][m]      = (double)B[i * n + m];
+      tab.Binv[i][m]   = (double)Bi[i * n + m];
+      tab.Blo[i][m]    = (double)(B[i * n + m] - (ld)tab.B[i][m]);
+      tab.Binvlo[i][m] = (double)(Bi[i * n + m] - (ld)tab.Binv[i][m]);
+    }
+  // barycentric differentiation matrix on the Gauss nodes
+  ld lam[GMAXN];
+  for (int j = 0; j < n; ++j)
+  {
+    ld pr = 1.0L;
+    for (int k = 0; k < n; ++k)
+      if (k != j)
+        pr *= (g[j] - g[k]);
+    lam[j] = 1.0L / pr;
+  }
+  for (int i = 0; i < n; ++i)
+  {
+    ld diag = 0.0L;
+    for (int m = 0; m < n; ++m)
+    {
+      if (m == i)
+        continue;
+      ld d = (lam[m] / lam[i]) / (g[i] - g[m]);
+      tab.Dg[i][m] = (double)d;
+      diag -= d;
+    }
+    tab.Dg[i][i] = (double)diag;
+  }
+  for (int e = 0; e < 2; ++e)
+    for (int m = 0; m < n; ++m)
+    {
+      tab.L[e][m]  = (double)lagz(m, n, g, (ld)e);
+      tab.Ld[e][m] = (double)lagzd(m, n, g, (ld)e);
+    }
+}
+
+// Geometry-order tables of order q at the Gauss points of order p (QTab
+// fields; T is QTab or GenTab).
+template <typename T>
+static void fill_q(int p, int q, T& tab)
+{
+  const int n = p + 1;
+  ld g[GMAXN], w[GMAXN];
+  gauss_legendre01(n, g, w);
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c <= q; ++c)
+    {
+      tab.Bq[i][c]  = (double)lag(c, q, g[i]);
+      tab.Bqd[i][c] = (double)lagd(c, q, g[i]);
+    }
+  for (int e = 0; e < 2; ++e)
+    for (int c = 0; c <= q; ++c)
+    {
+      tab.Be[e][c]  = (double)lag(c, q, (ld)e);
+      tab.Bde[e][c] = (double)lagd(c, q, (ld)e);
+    }
+}
+
 // Fills the three P tables (p = 1..3) and nine (P, Q) tables.
 void build_tables(PTab* pt, QTab* qt)
 {
@@ -165,73 +242,23 @@ void build_tables(PTab* pt, QTab* qt)
   memset(qt, 0, sizeof(QTab) * 9);
   for (int p = 1; p <= 3; ++p)
   {
-    int n = p + 1;
-    ld g[MAXN], w[MAXN];
-    gauss_legendre01(n, g, w);
-    PTab& T = pt[p - 1];
-    ld B[MAXN * MAXN], Bi[MAXN * MAXN];
-    for (int i = 0; i < n; ++i)
-    {
-      T.g[i] = (double)g[i];
-      T.w[i] = (double)w[i];
-      for (int m = 0; m < n; ++m)
-        B[i * n + m] = lag(m, p, g[i]);
-    }
-    invert_small(n, B, Bi);
-    for (int i = 0; i < n; ++i)
-      for (int m = 0; m < n; ++m)
-      {
-        T.B[i][m]      = (double)B[i * n + m];
-        T.Binv[i][m]   = (double)Bi[i * n + m];
-        T.Blo[i][m]    = (double)(B[i * n + m] - (ld)T.B[i][m]);
-        T.Binvlo[i][m] = (double)(Bi[i * n + m] - (ld)T.Binv[i][m]);
-      }
-    // barycentric differentiation matrix on the Gauss nodes
-    ld lam[MAXN];
-    for (int j = 0; j < n; ++j)
-    {
-      ld pr = 1.0L;
-      for (int k = 0; k < n; ++k)
-        if (k != j)
-          pr *= (g[j] - g[k]);
-      lam[j] = 1.0L / pr;
-    }
-    for (int i = 0; i < n; ++i)
-    {
-      ld diag = 0.0L;
-      for (int m = 0; m < n; ++m)
-      {
-        if (m == i)
-          continue;
-        ld d = (lam[m] / lam[i]) / (g[i] - g[m]);
-        T.Dg[i][m] = (double)d;
-        diag -= d;
-      }
-      T.Dg[i][i] = (double)diag;
-    }
-    for (int e = 0; e < 2; ++e)
-      for (int m = 0; m < n; ++m)
-      {
-        T.L[e][m]  = (double)lagz(m, n, g, (ld)e);
-        T.Ld[e][m] = (double)lagzd(m, n, g, (ld)e);
-      }
+    fill_p(p, pt[p - 1]);
     for (int q = 1; q <= 3; ++q)
-    {
-      QTab& U = qt[(p - 1) * 3 + (q - 1)];
-      for (int i = 0; i < n; ++i)
-        for (int c = 0; c <= q; ++c)
-        {
-          U.Bq[i][c]  = (double)lag(c, q, g[i]);
-          U.Bqd[i][c] = (double)lagd(c, q, g[i]);
-        }
-      for (int e = 0; e < 2; ++e)
-        for (int c = 0; c <= q; ++c)
-        {
-          U.Be[e][c]  = (double)lag(c, q, (ld)e);
-          U.Bde[e][c] = (double)lagd(c, q, (ld)e);
-        }
-    }
+      fill_q(p, q, qt[(p - 1) * 3 + (q - 1)]);
   }
+}
+
+// Tables of the general-order kernels: every 0 <= p <= 8, 1 <= q <= 8.
+void build_gen_tables(GenTab* gt)
+{
+  memset(gt, 0, sizeof(GenTab) * 72);
+  for (int p = 0; p <= 8; ++p)
+    for (int q = 1; q <= 8; ++q)
+    {
+      GenTab& t = gt[p * 8 + q - 1];
+      fill_p(p, t);
+      fill_q(p, q, t);
+    }
 }
 
 }  // namespace host

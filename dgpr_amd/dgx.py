@@ -537,6 +537,8 @@ class Solver:
         context launches."""
         w = self.stage_waves if waves is None else waves
         t = self.tile if tile is None else tile
+        if t == 2:
+            return "dgx_gen_stage_kernel<%d,double>" % self.mesh.p
         if t:
             return "dgx_tile_stage_kernel<%d,%d,%d,%d>" % (self.mesh.p, self.mesh.q,
                                                            self.TILE_ROWS[self.mesh.p], w)
@@ -545,7 +547,9 @@ class Solver:
     @property
     def tile(self):
         """Stage tile: 0 (one row of elements per wavefront, mixed-direction
-        face passes) or 1 (tiled, one direction per face subpass)."""
+        face passes), 1 (tiled, one direction per face subpass) or 2 (the
+        general-order kernel, one workgroup per element: the only one
+        outside 1 <= p, q <= 3, and available at every order)."""
         t = C.c_int()
         _chk(lib().dgx_get_tile(self.ctx, C.byref(t)), self.ctx)
         return t.value

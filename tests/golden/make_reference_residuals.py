@@ -122,6 +122,16 @@ CASES = {
     "single_element_p3": lambda: periodic_box_case((1, 1, 1), 3, 62),
     "ragged_1x2x1_p3": lambda: periodic_box_case((1, 2, 1), 3, 63),
     "ragged_2x1x3_p1": lambda: periodic_box_case((2, 1, 3), 1, 64),
+    # orders outside 1..3 (p = 0 .. 8, q up to 8): the general-order kernels
+    "box_4x3x2_p0": lambda: periodic_box_case((4, 3, 2), 0, 71),
+    "channel_p0": lambda: cases.channel_case(4, 4, 2, p=0, seed=72, nsteps=3),
+    "tgv2_p4": lambda: cases.tgv_case(n=2, p=4, seed=73, nsteps=3),
+    "bump_default_p4q2": lambda: bump_case(4, 2, DEFAULT_BOUNDS, 74),
+    "channel_p5": lambda: cases.channel_case(2, 3, 2, p=5, seed=75, nsteps=2),
+    "channel_q4_p2": lambda: cases.channel_case(2, 3, 2, p=2, q=4, seed=76, nsteps=2),
+    "channel_q3_p6": lambda: cases.channel_case(2, 2, 2, p=6, q=3, seed=77, nsteps=2),
+    "box_1x2x1_p7": lambda: periodic_box_case((1, 2, 1), 7, 78),
+    "channel_q8_p8": lambda: cases.channel_case(1, 2, 1, p=8, q=8, seed=79, nsteps=1),
 }
 
 

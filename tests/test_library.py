@@ -150,7 +150,9 @@ def test_create_rejects_bad_input():
     with pytest.raises(dgx.DgxError) as ei:
         dgx.Solver(bad, c.params)
     assert ei.value.status == 1
-    hi = dgx.Mesh(4, 1, 2, 2, 2, [-2] * 6, c.mesh.nodes)
+    # beyond the reference's MAX_ORDER = 8 (constants.cpp:21); checked before
+    # any device is touched
+    hi = dgx.Mesh(9, 1, 2, 2, 2, [-2] * 6, c.mesh.nodes)
     with pytest.raises(dgx.DgxError) as ei:
         dgx.Solver(hi, c.params)
     assert ei.value.status == 2

@@ -1,12 +1,15 @@
 """Pins the oracle against the reference's own outputs.
 
-``tests/golden/reference_residuals.npz`` holds, for 23 cases (BASELINE
+``tests/golden/reference_residuals.npz`` holds, for 32 cases (BASELINE
 config 0 at full size, periodic Taylor-Green p = 1..3 up to 8^3 p = 3 at
 Mach 0.1, curved q = 1..3
 channels with no-slip and inviscid walls, no-slip walls under a moving
 state, the reference's default bump with subsonic inflow / outflow and
 inviscid walls, free-stream boundaries with a source term, single-element
-and ragged periodic meshes), the reference's residual R and f = -Minv R at
+and ragged periodic meshes, and the orders outside 1..3: p = 0 periodic
+and walled, p = 4 Taylor-Green and default bump with q = 2, p = 5 and p = 2
+q = 4 channels, a p = 6 q = 3 channel, a periodic p = 7 box and a p = 8
+q = 8 channel), the reference's residual R and f = -Minv R at
 the initial state and its state and RMS(R) norms after N RK4 / TVD-RK3
 steps, produced by the reference's residual() (solve.cpp:150-807) and
 RK4_distributed / tvdRK3_distributed (time_distributed.cpp:46-102) compiled
