@@ -52,6 +52,12 @@ CASES = {
     "mean_run": ["-p", "2", "-q", "2", "-nx", "4", "-ny", "3", "-nz", "2", "-a", "0.4",
                  "-turb", "-scheme", "RK4", "-dt", "2e-4", "-niter", "3", "-mean_start", "1",
                  "-chkint", "1000"],
+    # orders outside 1..3: the p = 0 turbulent start, and three RK4 steps at
+    # p = 4 on q = 2 geometry
+    "p0_turb": ["-p", "0", "-q", "1", "-nx", "4", "-ny", "3", "-nz", "2", "-turb",
+                "-niter", "0"],
+    "p4_run": ["-p", "4", "-q", "2", "-nx", "3", "-ny", "2", "-nz", "2", "-turb", "-scheme",
+               "RK4", "-dt", "2e-4", "-niter", "3", "-chkint", "1000"],
 }
 
 

@@ -189,6 +189,29 @@ def test_driver_running_mean_and_restart(orc, driver, tmp_path):
     assert rel(outs2["state"], states[5]) < TOL and rel(outs2["mean"], m) < TOL
 
 
+def test_driver_general_order_matches_reference_driver(driver, tmp_path):
+    """dgx_drive -p 4 -q 2 (the general-order kernels) against the
+    reference's own CPU driver run of the same options
+    (tests/golden/reference_driver_outputs.*, case p4_run: turbulent start on
+    a 3x2x2 bump channel, three RK4 steps): same norms, same final state
+    (1e-10 relative)."""
+    import json
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    meta = json.load(open(os.path.join(gdir, "reference_driver_outputs.json")))["cases"]["p4_run"]
+    z = np.load(os.path.join(gdir, "reference_driver_outputs.npz"), allow_pickle=False)
+    out = str(tmp_path / "p4")
+    r, norms = run(driver, meta["args"] + ["-ofile", out])
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref_norms = z["p4_run/norms"]
+    assert sorted(norms) == list(range(len(ref_norms)))
+    en = max(abs(norms[t] - ref_norms[t]) / ref_norms[t] for t in range(len(ref_norms)))
+    ts, mesh, outs = dgx.read_dg_outputs(out + ".dg")
+    eU = rel(outs["state"], z["p4_run/state"])
+    print("  norms %.2e  state %.2e" % (en, eU), flush=True)
+    assert ts == meta["tstep"] and list(outs) == meta["keys"] and mesh.p == 4 and mesh.q == 2
+    assert en < TOL and eU < TOL
+
+
 def test_driver_matches_reference_driver_with_mean(driver, tmp_path):
     """The options of the reference's own CPU driver run held in
     tests/golden/reference_driver_outputs.* (solver/dg.cpp: turbulent start on

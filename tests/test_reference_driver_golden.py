@@ -5,7 +5,8 @@ tests/golden/make_reference_driver_outputs.py):
 * dgx_bump_nodes reproduces generate_bump_nodes (geometry_common.cpp:494-576)
   bit for bit, with and without wall-normal stretching, at q = 2 and 3;
 * the initial states: uniform (solver_init.cpp:22-28) exactly, turbulent
-  (solver_init.cpp:30-73) to 1e-15 of its scale;
+  (solver_init.cpp:30-73) to 1e-15 of its scale, also at p = 0 (one node
+  per element, at the reference origin);
 * dgx_write_dg_outputs, given the driver's "state" and "mean", writes the
   driver's file byte for byte (io.cpp:33-89, outputs in the map's order).
 The GPU driver's run of the same options is compared in
@@ -59,7 +60,7 @@ def test_bump_nodes_match_reference_bitwise(name):
     assert np.array_equal(m.nodes, ref), float(np.max(np.abs(m.nodes - ref)))
 
 
-@pytest.mark.parametrize("name", ["bump_default", "bump_stretched_turb"])
+@pytest.mark.parametrize("name", ["bump_default", "bump_stretched_turb", "p0_turb"])
 def test_initial_state_matches_reference(name):
     d, m = mesh_of(name)
     prm = dgx.make_reference_state()
