@@ -541,8 +541,10 @@ int dgx::stage_plan(const dgx_ctx* c, int scheme, double dt, StageArgs sa[4])
 extern "C" {
 
 // Launches one step; if want_norm, stores sum(R^2) of stage 1 in c->sum
-// (RK4: from the accumulator, which holds f1; TVD-RK3: from f written by the
-// first stage), or, with norm_dev, the step's RMS(R) in *norm_dev.
+// (the tiled kernel computes the per-block sums in its epilogue; otherwise
+// they come from the accumulator, which holds f1 in RK4, or from f written by
+// the first stage in TVD-RK3), or, with norm_dev, the step's RMS(R) in
+// *norm_dev.
 static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm,
                               double* norm_dev = nullptr)
 {
@@ -552,9 +554,12 @@ static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm,
   c->tu_valid = false;  // until the last stage has rewritten them
   StageArgs sa[4];
   int ns = stage_plan(c, scheme, dt, sa);
+  const bool fused = want_norm && c->stage_tile == 1;
   for (int k = 0; k < ns; ++k)
   {
-    if (k == 0 && want_norm && scheme != DGX_RK4)
+    if (k == 0 && fused)
+      sa[0].normp = c->partial;
+    else if (k == 0 && want_norm && scheme != DGX_RK4)
       sa[0].fout = c->fs;
     if ((e = rhs_dispatch(c->p, c->q, mesh_args(c), c->phys, sa[k], c->lam, c->status,
                           c->stage_waves, c->stream, c->stage_tile)) != hipSuccess)
@@ -562,10 +567,10 @@ static hipError_t launch_step(dgx_ctx* c, int scheme, double dt, bool want_norm,
     if (k == 0 && want_norm)
     {
       const double* f1 = (scheme == DGX_RK4) ? c->acc : c->fs;
-      if ((e = norm_partials_dispatch(c->p, c->q, mesh_args(c), f1, c->partial, c->stream)) !=
-          hipSuccess)
+      if (!fused && (e = norm_partials_dispatch(c->p, c->q, mesh_args(c), f1, c->partial,
+                                                c->stream)) != hipSuccess)
         return e;
-      const int nb = norm_blocks(c->p, c->q, c->nelem);
+      const int nb = fused ? tile_blocks(c->p, c->nelem) : norm_blocks(c->p, c->q, c->nelem);
       e = norm_dev ? norm_dispatch(c->partial, nb, 1.0 / (double)c->nstate, norm_dev, c->stream)
                    : sum_dispatch(c->partial, nb, c->sum, c->stream);
       if (e != hipSuccess)

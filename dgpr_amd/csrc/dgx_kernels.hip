@@ -516,9 +516,16 @@ hipError_t launch_tile_ct(const MeshArgs& m, const Phys& ph, const StageArgs& sa
   constexpr int R = tile_rows(P);
   constexpr int E = TileDims<P, R>::E;
   int nblk = (m.nelem + E - 1) / E;
-  if (waves == 2)
+  const bool norm = sa.normp != nullptr;
+  if (waves == 2 && norm)
+    hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 2, TT, AFF, CT, true>), dim3(nblk),
+                       dim3(64), 0, st, m, ph, sa, lam, status);
+  else if (waves == 2)
     hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 2, TT, AFF, CT>), dim3(nblk), dim3(64), 0,
                        st, m, ph, sa, lam, status);
+  else if (waves == 3 && norm)
+    hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 3, TT, AFF, CT, true>), dim3(nblk),
+                       dim3(64), 0, st, m, ph, sa, lam, status);
   else if (waves == 3)
     hipLaunchKernelGGL((dgx_tile_stage_kernel<P, Q, R, 3, TT, AFF, CT>), dim3(nblk), dim3(64), 0,
                        st, m, ph, sa, lam, status);
@@ -756,6 +763,13 @@ hipError_t inviscid_rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph
 #define CALL_INV(P, Q) launch_inviscid<P, Q>(m, ph, V, Tin, fout, lam, status, st)
   DGX_DISPATCH_PQ(p, q, CALL_INV)
 #undef CALL_INV
+}
+
+int tile_blocks(int p, int nelem)
+{
+  const int rows = tile_rows(p);
+  const int e = rows > 0 ? rows * (64 / ((p + 1) * (p + 1) * (p + 1))) : 1;
+  return (nelem + e - 1) / e;
 }
 
 int stage_blocks(int p, int nelem)

@@ -97,6 +97,10 @@ struct StageArgs
   int trace32;
   int ctype;  // volume contractions of the tiled p = 3 kernel: 0 FP64, 1 FP32
               // matrix cores, 2 split-BF16 matrix cores (with trace32 set)
+  // tiled kernel only (tile 1): when set, normp[block] receives the block's
+  // sum of R^2, R = -B3^T W f (the residual norm of this stage's f), for
+  // tile_blocks(p, nelem) blocks
+  double* normp = nullptr;
 };
 
 // One member of a batched stage launch: its mesh, physics, stage arguments
@@ -128,6 +132,8 @@ hipError_t rhs_batch_dispatch(int p, int q, const BatchItem* items_dev, int nite
                               int* status, int waves, bool affine, hipStream_t st);
 // grid blocks of one mesh in a stage launch
 int stage_blocks(int p, int nelem);
+// grid blocks of the tiled kernel (tile 1) on one mesh
+int tile_blocks(int p, int nelem);
 // fout = f(V; mu = 0), the inviscid right-hand side, from V and its trace
 // records Tin (written with any mu: their tractions are not read)
 hipError_t inviscid_rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,

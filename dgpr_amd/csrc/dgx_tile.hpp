@@ -45,7 +45,10 @@ struct TileDims
   static constexpr int FPS  = 64 / NF;            // whole faces per subpass
   static constexpr int NSUB = (2 * E + FPS - 1) / FPS;  // subpasses per direction
   static constexpr int FX   = 9;                  // doubles per face point exchanged
-  static constexpr int SR   = (5 * 64 > FX * 64) ? 5 * 64 : FX * 64;
+  // exchange region: a face subpass (FX per lane); the residual norm uses
+  // 6 per lane (one row's W f_g and the lane's running sum of R^2)
+  static constexpr int SR   = FX * 64;
+  static_assert(FX >= 6, "the residual norm needs 6 doubles per lane");
 };
 
 // tangential directions of faces normal to direction d (in increasing order)
@@ -351,6 +354,41 @@ __device__ __forceinline__ void tile_trace_direction(
                                             Tout, ok);
 }
 
+// One pass of R = -B3^T W f_g over one row of a tile: s[r][node] <-
+// sum_m B[m][i_c] s[r][node with i_c = m] along direction c within each
+// element (s holds 5 x 64 doubles, node = lane); every lane calls (two
+// barriers).
+template <int N, int NB>
+__device__ __forceinline__ void tile_modal_pass(double* s, const double (*B)[MAXN], int c,
+                                                const int (&idx)[3], bool act)
+{
+  const int tid  = threadIdx.x;
+  const int base = (tid / NB) * NB;
+  double out[5];
+  __syncthreads();
+  if (act)
+  {
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+    {
+      double a = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m)
+      {
+        const int nn = (c == 0) ? nid<N>(m, idx[1], idx[2])
+                     : ((c == 1) ? nid<N>(idx[0], m, idx[2]) : nid<N>(idx[0], idx[1], m));
+        a += B[m][idx[c]] * s[r * 64 + base + nn];
+      }
+      out[r] = a;
+    }
+  }
+  __syncthreads();
+  if (act)
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+      s[r * 64 + tid] = out[r];
+}
+
 // LDS of one tile, carved from a byte buffer (see StageLds).
 template <int P, int Q, int R>
 struct TileLds
@@ -375,8 +413,11 @@ struct TileLds
 // (TileLds<P, Q, R>::bytes, 16-byte aligned).
 // CT selects how the volume contractions are done: 0 in FP64 on the vector
 // ALUs; at p = 3, 1 in FP32 and 2 in split BF16 on the matrix cores
-// (dgx_mfma.hpp), the flux physics between them staying FP64.
-template <int P, int Q, int R, typename TT, bool AFF, int CT = 0>
+// (dgx_mfma.hpp), the flux physics between them staying FP64.  NORM: the
+// instance that also writes the tile's sum of R^2 to sa.normp (the first
+// stage of a step with its residual norm); kept apart so that the other
+// stages run code without it.
+template <int P, int Q, int R, typename TT, bool AFF, int CT = 0, bool NORM = false>
 __device__ __forceinline__ void tile_body_lds(const MeshArgs& mesh, const Phys& ph,
                                               const StageArgs& sa,
                                               const double* __restrict__ lam, int* status,
@@ -549,13 +590,16 @@ __device__ __forceinline__ void tile_body_lds(const MeshArgs& mesh, const Phys& 
   tile_face_direction<P, Q, R, 2, TT, AFF>(mesh, ph, pt, qt, Tin, lam, sX, sG, sNb, sR, e0, acc,
                                            ok);
 
-  // ---- form f_g and the RK epilogue, one row at a time ----
+  // ---- form f_g and the RK epilogue, one row at a time; with NORM, each
+  // row's R = -B3^T W f_g in sR (three passes) and the lane's running sum of
+  // R^2 in sR[5 * 64 + lane] ----
 #pragma unroll
   for (int rr = 0; rr < R; ++rr)
   {
     const int le = rr * EPR + tid / NB;
     const int e  = e0 + le;
-    if (tid < EPR * NB && e < mesh.nelem)
+    const bool act = tid < EPR * NB && e < mesh.nelem;
+    if (act)
     {
       const double iwJ = rcp(wJ[rr]);
       const size_t base = (size_t)e * 5 * NB + nd0;
@@ -564,6 +608,8 @@ __device__ __forceinline__ void tile_body_lds(const MeshArgs& mesh, const Phys& 
       {
         const size_t ix = base + (size_t)r * NB;
         const double fv = ph.source[r] - acc[rr][r] * iwJ;
+        if (NORM)
+          sR[r * 64 + tid] = -wJ[rr] * fv;
         if (sa.fout)
           sa.fout[ix] = fv;
         double ft = fv;
@@ -585,6 +631,32 @@ __device__ __forceinline__ void tile_body_lds(const MeshArgs& mesh, const Phys& 
         }
       }
     }
+    if constexpr (NORM)
+    {
+      // the modal residual whose RMS the step reports, one direction per
+      // pass; only this lane reads and writes its running-sum slot
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        tile_modal_pass<N, NB>(sR, pt.B, c, idx, act);
+      double s2 = 0.0;
+      if (act)
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          s2 += sR[r * 64 + tid] * sR[r * 64 + tid];
+      sR[5 * 64 + tid] = (rr == 0) ? s2 : sR[5 * 64 + tid] + s2;
+      __syncthreads();
+    }
+  }
+  if constexpr (NORM)
+  {
+    // the tile's sum of R^2: a butterfly over the wavefront (fixed order:
+    // deterministic)
+    double sq = sR[5 * 64 + tid];
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1)
+      sq += __shfl_xor(sq, w, 64);
+    if (tid == 0)
+      sa.normp[block] = sq;
   }
 
   // ---- face traces of the new state for the next stage ----
@@ -603,11 +675,11 @@ __device__ __forceinline__ void tile_body_lds(const MeshArgs& mesh, const Phys& 
     atomicOr(status, 1);
 }
 
-template <int P, int Q, int R, int W, typename TT, bool AFF, int CT = 0>
+template <int P, int Q, int R, int W, typename TT, bool AFF, int CT = 0, bool NORM = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 void dgx_tile_stage_kernel(MeshArgs mesh, Phys ph, StageArgs sa, const double* __restrict__ lam,
                            int* status)
 {
   __shared__ __attribute__((aligned(16))) unsigned char lds[TileLds<P, Q, R>::bytes];
-  tile_body_lds<P, Q, R, TT, AFF, CT>(mesh, ph, sa, lam, status, blockIdx.x, lds);
+  tile_body_lds<P, Q, R, TT, AFF, CT, NORM>(mesh, ph, sa, lam, status, blockIdx.x, lds);
 }
