@@ -271,9 +271,10 @@ TILE_CASES = ["bump_freestream_p2q1", "channel_inviscid_p2", "channel_p1", "chan
 @pytest.mark.parametrize("name", TILE_CASES)
 def test_tiled_kernel_parity(name, orc, built):
     """The tiled stage kernel (several rows of elements per wavefront, faces
-    one direction at a time) on every q = 1 case: the residual and the
-    state after N steps match the oracle as the one-row kernel's do, and the
-    two kernels agree with each other to rounding."""
+    one direction at a time) on every q = 1 case: the residual, the state
+    after N steps and the per-step norms (which the tiled kernel computes in
+    its first stage's epilogue) match the oracle as the one-row kernel's do,
+    and the two kernels agree with each other to rounding."""
     case = CASES[name]()
     m = case.mesh
     assert m.q == 1
