@@ -20,7 +20,8 @@ rank, reads the .dg file it writes and stores, as data:
                                               bytes, for the layout check)
   tests/golden/reference_driver_outputs.json  per case: options, tstep, keys
 
-The GPU box never runs this script.
+The script needs the reference sources, so it runs only where they are;
+the tests load the fixtures.
 """
 import json
 import os

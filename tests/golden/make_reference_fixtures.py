@@ -10,7 +10,8 @@ as text and writes only numbers:
   tests/golden/reference_quadrature.json  quadrature_test.cpp:177-244: rule
                                        arguments and expected integrals
 
-The GPU box never runs this script; tests load the committed fixtures.
+The script needs the reference sources, so it runs only where they are;
+tests load the committed fixtures.
 """
 import json
 import os

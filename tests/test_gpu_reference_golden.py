@@ -2,7 +2,8 @@
 
 For every case of ``tests/golden/reference_residuals.npz`` (the reference's
 residual() and RK4 / TVD-RK3 steps run on the same mesh and initial state;
-BASELINE config 0 at full size among them) the device evaluates the
+the 8^3 p = 2 Taylor-Green workload at full size among them) the device
+evaluates the
 residual and takes the same steps through the C ABI:
 
 * state after N steps and every step's RMS(R): 1e-10 relative;
@@ -12,7 +13,7 @@ residual and takes the same steps through the C ABI:
   oracle/liboracle_x.so).  On the Mach 0.1 Taylor-Green meshes the residual
   is ill-conditioned: the mass row cancels O(|m|/h) terms down to O(M^2 |m|
   /h), and the reference's double result is itself ~1e-12 from the exact
-  value (1.03e-12); two double evaluations in different
+  value (8^3 p = 2 Taylor-Green: 1.03e-12); two double evaluations in different
   orders of operations cannot agree beyond their rounding floors there.  For
   those cases the bound is relative to that floor: the device must be within
   2x the reference's own distance from the exact value, and within 3x of it

@@ -1,4 +1,4 @@
-"""The BASELINE workloads at their full sizes, through the C ABI.
+"""The benchmark workloads at their full sizes, through the C ABI.
 
 Parity against the oracle needs meshes the oracle finishes in seconds; these
 tests run the benchmark configurations themselves and check properties that

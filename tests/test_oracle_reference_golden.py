@@ -1,7 +1,7 @@
 """Pins the oracle against the reference's own outputs.
 
-``tests/golden/reference_residuals.npz`` holds, for 32 cases (BASELINE
-config 0 at full size, periodic Taylor-Green p = 1..3 up to 8^3 p = 3 at
+``tests/golden/reference_residuals.npz`` holds, for 32 cases (the 8^3 p = 2
+Taylor-Green workload at full size, periodic Taylor-Green p = 1..3 up to 8^3 p = 3 at
 Mach 0.1, curved q = 1..3
 channels with no-slip and inviscid walls, no-slip walls under a moving
 state, the reference's default bump with subsonic inflow / outflow and
