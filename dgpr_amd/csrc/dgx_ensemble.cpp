@@ -15,6 +15,8 @@
 //  * batched (layout 0): one launch of the batched one-row stage kernel per
 //    (p, q, affine) group (each grid block finds its member in a sorted
 //    item table).
+// Members of an order outside 1 <= p, q <= 3 run one launch of the
+// general-order stage kernel each, in every layout.
 // One RK step of the whole ensemble can be captured in a hipGraph and
 // replayed (kernels only).  The autotuner picks, per launch group, the
 // occupancy target and, for the ensemble kernel, the tile (one-row units or
@@ -45,7 +47,11 @@ struct EnsGroup
   int nblocks;   // grid blocks (batched) or work units (ensemble kernel)
   int waves;
   int tile;      // ensemble kernel: 0 one-row units, 1 tiled units
+  bool general;  // members of an order outside 1 <= p, q <= 3: one launch of
+                 // the general-order kernel per member (host items), in
+                 // every layout
   BatchItem* items[4];  // device item tables, one per stage
+  std::vector<BatchItem> host[4];  // the same on the host (general groups)
 };
 }  // namespace
 
@@ -102,6 +108,8 @@ static void drop_graph(dgx_ensemble* en)
 // Whether group g runs in the current layout.
 static bool active(const dgx_ensemble* en, const EnsGroup& g)
 {
+  if (g.general)
+    return true;
   if (en->layout == 0)
     return !g.unit_grid;
   if (!g.unit_grid)
@@ -150,9 +158,13 @@ static dgx_status plan(dgx_ensemble* en, int scheme, const double* dt)
         it.p = c->p;
         host[s].push_back(it);
       }
-      block0 += g.unit_grid ? ens_units(c->p, c->nelem, g.tile) : stage_blocks(c->p, c->nelem);
+      block0 += g.general ? c->nelem
+                          : (g.unit_grid ? ens_units(c->p, c->nelem, g.tile)
+                                         : stage_blocks(c->p, c->nelem));
     }
     g.nblocks = block0;
+    for (int s = 0; s < 4; ++s)
+      g.host[s] = g.general ? host[s] : std::vector<BatchItem>();
     EHIP(en, hipStreamSynchronize(en->stream));  // no launch may still read the tables
     for (int s = 0; s < ns; ++s)
       EHIP(en, hipMemcpy(g.items[s], host[s].data(), sizeof(BatchItem) * host[s].size(),
@@ -177,6 +189,17 @@ static hipError_t enqueue_stage(dgx_ensemble* en, int s, int only_group = -1)
     EnsGroup& g = en->groups[gi];
     if (!active(en, g))
       continue;
+    if (g.general)
+    {
+      for (const BatchItem& it : g.host[s])
+      {
+        hipError_t e = gen_rhs_dispatch(it.p, g.q, it.mesh, it.ph, it.sa, it.lam, en->status,
+                                        en->stream);
+        if (e != hipSuccess)
+          return e;
+      }
+      continue;
+    }
     hipError_t e = g.unit_grid
                        ? ens_stage_dispatch(g.items[s], (int)g.members.size(), g.nblocks,
                                             en->status, g.waves, g.tile, g.affine != 0, g.p,
@@ -238,12 +261,6 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
       return st;
     }
     en->ctx.push_back(c);
-    if (general_order(c->p, c->q))
-    {
-      efail(nullptr, DGX_ERR_UNSUPPORTED, "member %d: ensembles cover 1 <= p, q <= 3", i);
-      dgx_ensemble_destroy(en);
-      return DGX_ERR_UNSUPPORTED;
-    }
   }
   en->device = en->ctx[0]->device;
   en->layout = 1;
@@ -259,6 +276,7 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
     ng.nblocks = 0;
     ng.waves = unit_grid ? 3 : default_stage_waves(p, q);
     ng.tile = 0;
+    ng.general = !unit_grid && general_order(p, q);
     for (int s = 0; s < 4; ++s)
       ng.items[s] = nullptr;
     en->groups.push_back(ng);
@@ -281,7 +299,7 @@ dgx_status dgx_ensemble_create(int ncase, const dgx_mesh* meshes, const dgx_para
   for (int i : order)
   {
     dgx_ctx* c = en->ctx[i];
-    if (c->q == 1)
+    if (c->q == 1 && !general_order(c->p, c->q))
     {
       group_of(true, 0, 1, c->affine).members.push_back(i);
       group_of(true, c->p, 1, c->affine).members.push_back(i);
@@ -450,7 +468,7 @@ dgx_status dgx_ensemble_autotune(dgx_ensemble* en, int scheme, const double* dt,
     for (int gi = 0; gi < (int)en->groups.size(); ++gi)
     {
       EnsGroup& g = en->groups[gi];
-      if (!active(en, g))
+      if (!active(en, g) || g.general)
         continue;
       int best_w = g.waves, best_t = g.tile;
       float best_ms = 0.0f;
@@ -548,7 +566,7 @@ dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* en, int i, int* layout
       if (layout)
         *layout = g.unit_grid ? en->layout : 0;
       if (tile)
-        *tile = g.unit_grid ? g.tile : 0;
+        *tile = g.general ? 2 : (g.unit_grid ? g.tile : 0);
       if (waves)
         *waves = g.waves;
       return DGX_OK;

@@ -95,8 +95,8 @@ typedef struct dgx_ctx dgx_ctx;
  * 819-830).  1 <= p, q <= 3 run the tuned kernels; a q = 1 mesh there whose
  * elements are all parallelepipeds runs the constant-Jacobian kernels
  * (chosen here, nothing to set).  The other orders run the general-order
- * kernels (stage tile 2, one workgroup per element); the adjoint and
- * ensembles cover 1 <= p, q <= 3 and return DGX_ERR_UNSUPPORTED outside. */
+ * kernels (stage tile 2, one workgroup per element); the adjoint covers
+ * 1 <= p, q <= 3 and returns DGX_ERR_UNSUPPORTED outside. */
 dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params,
                       dgx_ctx** out);
 
@@ -229,7 +229,8 @@ dgx_status dgx_set_mean(dgx_ctx* ctx, const double* mean_host);
  * dgx_ensemble_set_layout(ens, 2) that kernel runs once per (order, affine)
  * class of the q = 1 members instead, and with dgx_ensemble_set_layout(ens,
  * 0) every (p, q, affine) group runs its own batched launch of the one-row
- * kernel.  With use_graph != 0 one step of the whole ensemble is captured
+ * kernel.  Members of an order outside 1 <= p, q <= 3 run one launch of the
+ * general-order kernel each, in every layout.  With use_graph != 0 one step of the whole ensemble is captured
  * once in a hipGraph and replayed.  Results equal those of dgx_step on each
  * member (the same arithmetic, to rounding) in every
  * layout.  The call only enqueues work on the ensemble's stream: call dgx_ensemble_sync before
@@ -263,8 +264,8 @@ dgx_status dgx_ensemble_autotune(dgx_ensemble* ens, int scheme, const double* dt
  * (p, q, affine) group. */
 dgx_status dgx_ensemble_set_layout(dgx_ensemble* ens, int layout);
 /* The launch member i runs in: layout (0 for a batched launch, else the
- * layout 1 or 2 of dgx_ensemble_set_layout), tile (0/1) and occupancy
- * target (waves per SIMD). */
+ * layout 1 or 2 of dgx_ensemble_set_layout), tile (0/1; 2 for the
+ * general-order kernel) and occupancy target (waves per SIMD). */
 dgx_status dgx_ensemble_member_tuning(const dgx_ensemble* ens, int i, int* layout, int* tile,
                                       int* waves);
 const char* dgx_ensemble_last_error(const dgx_ensemble* ens);

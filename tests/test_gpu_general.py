@@ -8,8 +8,8 @@ constants.cpp:21, geometry_distributed.cpp:819-830).
 * Here: the same kernels at the tuned orders against the tuned kernels
   (same trace records, same algebra: agreement to rounding), the stepping
   and the per-step norms against the oracle on curved meshes with walls,
-  FP32 trace records, the sampled outputs, the tile rules, and the paths
-  that refuse these orders (adjoint, ensembles)."""
+  FP32 trace records, the sampled outputs, ensembles with members of these
+  orders, the tile rules, and the path that refuses them (the adjoint)."""
 import numpy as np
 import pytest
 
@@ -121,10 +121,46 @@ def test_general_orders_sample_output(orc, built, name, N):
             assert err < 1e-13, var
 
 
+@pytest.mark.parametrize("layout", [1, 2, 0])
+def test_ensemble_with_general_order_members(built, layout):
+    """Ensemble members of orders outside the tuned set (p = 4, p = 0, p = 3
+    with q = 4) next to tuned ones: in every layout, eagerly and replayed
+    from the captured graph, each member ends where its single-case steps
+    do (the general-order members run the same kernel: bit for bit)."""
+    cs = [GENERAL["channel_p4"](), GENERAL["tgv_p0"](), GENERAL["channel_q4_p3"](),
+          TUNED["tgv_curved_p2"](), TUNED["tgv_affine_p3"]()]
+    nsteps = 3
+    ref = []
+    for c in cs:
+        with dgx.Solver(c.mesh, c.params) as s:
+            if c.mesh.p in (1, 2, 3) and c.mesh.q <= 3:
+                s.tile = 0
+            s.set_state(c.U0)
+            s.step(c.dt, "RK4", nsteps, norms=False)
+            ref.append(s.get_state())
+    dts = [c.dt for c in cs]
+    with dgx.Ensemble([c.mesh for c in cs], [c.params for c in cs]) as en:
+        en.set_layout(layout)
+        for i in range(3):
+            assert en.member_tuning(i)[1] == 2
+        for graph in (False, True):
+            for m, c in zip(en.members, cs):
+                m.set_state(c.U0)
+            en.step("RK4", dts, nsteps, graph=graph)
+            for i, (m, r) in enumerate(zip(en.members, ref)):
+                e = rel(m.get_state(), r)
+                print("  layout %d graph %s member %d (p=%d q=%d) rel err %.3e"
+                      % (layout, graph, i, cs[i].mesh.p, cs[i].mesh.q, e), flush=True)
+                if i < 3:
+                    assert e == 0.0
+                else:
+                    assert e < 1e-14
+
+
 def test_general_order_rules(built):
     """Tile 2 is the only tile outside the tuned orders (0 and 1 refused);
-    occupancy targets and autotuning leave it alone; the adjoint and the
-    ensembles refuse these orders."""
+    occupancy targets and autotuning leave it alone; the adjoint refuses
+    these orders."""
     c = GENERAL["channel_p4"]()
     with dgx.Solver(c.mesh, c.params) as s:
         assert s.tile == 2
@@ -139,9 +175,6 @@ def test_general_order_rules(built):
         with pytest.raises(dgx.DgxError) as ei:
             s.residual_vjp(np.ones(c.mesh.state_size))
         assert ei.value.status == dgx.ERR_UNSUPPORTED
-    with pytest.raises(dgx.DgxError) as ei:
-        dgx.Ensemble([c.mesh], [c.params])
-    assert ei.value.status == dgx.ERR_UNSUPPORTED
     # p = 9 is beyond the reference's MAX_ORDER
     m = c.mesh
     with pytest.raises(dgx.DgxError) as ei:
