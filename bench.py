@@ -25,8 +25,10 @@ Prints one JSON line:
                one 8^3 block per host thread
 
 --workload config0/config1 run the 8^3 p=2 cube and the curved channel the
-same way; config3 the 64-case ensemble, config4 the adjoint.  Only one GPU
-is used: --gpus N > 1 exits with a message.
+same way; config3 the 64-case ensemble, config4 the adjoint; orders times
+every solution order 0..8 (periodic Taylor-Green cubes of 1.3-2.1 M DOF, the
+general-order kernel and, at p = 1..3, the tuned one).  Only one GPU is
+used: --gpus N > 1 exits with a message.
 """
 import argparse
 import ctypes as C
@@ -118,6 +120,45 @@ def cpu_baseline(seconds_hint=20.0, threads=None):
                       "blocks (512 elements, 32768 DOF each, dt 2.5e-4), one per host thread, "
                       "oracle (CPU restatement of the reference), %.1f s wall; setup %.2f s "
                       "per block not included" % (total_steps, nthr, el, setup)}
+
+
+def run_orders(a, h):
+    """Every solution order the reference accepts, p = 0..8, on periodic
+    Taylor-Green cubes of 1.3-2.1 M DOF (Mach 0.2, q = 1, RK4 dt 1e-4): the
+    general-order kernel (stage tile 2) at every order and the context's
+    default (tuned) kernel at p = 1..3, 10 timed steps each after 2 untimed
+    ones, HIP events on the library's stream, no per-step norm."""
+    steps = a.steps if a.steps_given else 10
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    h.hipEventCreate(C.byref(ev0))
+    h.hipEventCreate(C.byref(ev1))
+    sizes = {0: 128, 1: 64, 2: 40, 3: 32, 4: 24, 5: 20, 6: 16, 7: 14, 8: 12}
+    rows = []
+    for p, n in sizes.items():
+        case = cases.tgv_case(n=n, p=p, seed=5, M=0.2, dt=1e-4)
+        with dgx.Solver(case.mesh, case.params) as s:
+            stream = C.c_void_p(s.stream())
+            tiles = [None, 2] if p in (1, 2, 3) else [2]
+            for t in tiles:
+                if t is not None:
+                    s.tile = t
+                s.set_state(case.U0)
+                s.step_async(case.dt, "RK4", 2)
+                s.sync()
+                sec = timed_steps(h, s, case, steps, stream, ev0, ev1)
+                rows.append({"p": p, "mesh": "%d^3" % n, "dof": case.dof, "tile": s.tile,
+                             "kernel": s.stage_kernel_name(),
+                             "us_per_stage": 1e6 * sec / (steps * 4),
+                             "gdof_stage_s": case.dof * 4 * steps / sec / 1e9})
+    p4 = [r for r in rows if r["p"] == 4][0]
+    print(json.dumps({
+        "metric": "GDOF-stage-updates/s per solution order", "value": p4["gdof_stage_s"],
+        "unit": "GDOF-stage/s", "n_gpus": 1, "steps": steps, "warmup": 2,
+        "ms_per_step": 4e-3 * p4["us_per_stage"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded initial states)",
+        "config": {"workload": "orders_p0_to_p8", "value_is": "p = 4 (general-order kernel)",
+                   "parallelism": "single GPU"},
+        "orders": rows}), flush=True)
 
 
 def run_ensemble(a, h):
@@ -491,7 +532,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="config2",
-                    choices=["config0", "config1", "config2", "config3", "config4"])
+                    choices=["config0", "config1", "config2", "config3", "config4", "orders"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=None)
@@ -513,6 +554,8 @@ def main():
         return run_ensemble(a, h)
     if a.workload == "config4":
         return run_adjoint(a, h)
+    if a.workload == "orders":
+        return run_orders(a, h)
     case = {"config0": cases.config0, "config1": cases.config1, "config2": cases.config2}[a.workload]()
     if a.steps is None:
         a.steps = case.nsteps
