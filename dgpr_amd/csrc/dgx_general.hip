@@ -7,12 +7,15 @@
 //
 // Same method, record layout and stage algebra as the tuned kernels
 // (dgx_stage.hpp, which describes them); the mapping differs:
-//   * one workgroup per element, one Gauss node per lane: the (p+1)^3 nodes
-//     on round_up((p+1)^3, 64) lanes (64 up to p = 3, 128 at p = 4, 256 at
-//     p = 5, ..., 768 at p = 8), workgroup barriers between the phases;
-//   * faces in passes of up to six whole faces, one face point per lane;
-//   * the geometry order is a run-time loop bound, the element's nodes (up
-//     to 9^3 x 3 doubles) staged in dynamic LDS;
+//   * one Gauss node per lane, workgroup barriers between the phases: an
+//     element's (p+1)^3 nodes on round_up((p+1)^3, 64) lanes (64 up to
+//     p = 3, 128 at p = 4, 256 at p = 5, ..., 768 at p = 8), or at p <= 2,
+//     where an element has at most 27 nodes, up to 64 / (p+1)^3 elements
+//     per 64-lane workgroup (fewer when their geometry nodes would take
+//     more than 32 KB of LDS);
+//   * faces in passes of whole faces, one face point per lane;
+//   * the geometry order is a run-time loop bound, the elements' nodes (up
+//     to 9^3 x 3 doubles each) staged in dynamic LDS;
 //   * the 1D tables of every (p, q) sit in one device array (built on the
 //     host in long double) and are copied to LDS by each workgroup.
 // The solution order is a template parameter (register arrays and loops
@@ -36,14 +39,28 @@ struct GDims
   static constexpr int N   = P + 1;
   static constexpr int NB  = N * N * N;
   static constexpr int NF  = N * N;
-  static constexpr int BT  = NB <= 64 ? 64 : ((NB + 63) / 64) * 64;  // lanes per element
-  static constexpr int FPP = (BT / NF < 6) ? BT / NF : 6;            // faces per pass
-  static constexpr int NPASS = (6 + FPP - 1) / FPP;
+  // elements per workgroup at most (several when an element has <= 32 nodes)
+  static constexpr int EPB = NB <= 32 ? 64 / NB : 1;
+  static constexpr int BT  = NB <= 64 ? 64 : ((NB + 63) / 64) * 64;  // lanes per workgroup
+  // whole faces per pass, one face point per lane (with one element per
+  // workgroup at most its six faces)
+  static constexpr int FPP = (EPB > 1 || BT / NF < 6) ? BT / NF : 6;
+  static constexpr int NPASS1 = (6 + FPP - 1) / FPP;  // face passes of one element
   static constexpr int FS  = 17;  // doubles per face point in a face pass
   static constexpr int TS  = 9;   // doubles per face point while producing traces
-  static constexpr int RG0 = (5 * NB > FS * FPP * NF) ? 5 * NB : FS * FPP * NF;
-  static constexpr int RG  = (RG0 > TS * 6 * NF) ? RG0 : TS * 6 * NF;  // exchange region
+  static constexpr int RG0 = 5 * NB * EPB;  // volume divergence of every element
+  static constexpr int RG  = (RG0 > FS * FPP * NF) ? RG0 : FS * FPP * NF;  // exchange region
 };
+
+// Elements per workgroup for (p, q): the most that fit GDims::EPB and 32 KB
+// of geometry nodes in LDS.
+static int gen_epb_of(int epb_max, int q)
+{
+  const int n3 = (q + 1) * (q + 1) * (q + 1) * 3;
+  int e = 32768 / (n3 * (int)sizeof(double));
+  e = e < 1 ? 1 : e;
+  return e < epb_max ? e : epb_max;
+}
 
 __device__ __forceinline__ int pick3i(int k, int a, int b, int c)
 {
@@ -108,27 +125,33 @@ __device__ __forceinline__ double gen_face_geometry(const GenTab& t, const doubl
   return face_normal(J, nd, hi, nv);
 }
 
-// Face table of element e: sNb[f] the neighbour across face f, or -1 -
-// (boundary face index) with the boundary code in sBt[f].
-__device__ __forceinline__ void gen_face_table(const MeshArgs& mesh, int e, int* sNb, int* sBt)
+// Face table of the workgroup's epb elements from e0: sNb[l][f] the
+// neighbour across face f, or -1 - (boundary face index) with the boundary
+// code in sBt[l][f].
+__device__ __forceinline__ void gen_face_table(const MeshArgs& mesh, int e0, int epb,
+                                               int (*sNb)[6], int (*sBt)[6])
 {
-  const int f = threadIdx.x;
-  if (f < 6)
+  for (int x = threadIdx.x; x < epb * 6; x += blockDim.x)
   {
-    const int ci = e % mesh.nx, cj = (e / mesh.nx) % mesh.ny, ck = e / (mesh.nx * mesh.ny);
-    int bt = -1;
-    int nb = neighbour(mesh, ci, cj, ck, f, &bt);
-    if (nb < 0)
-      nb = -1 - bface_index(mesh, ci, cj, ck, f);
-    sNb[f] = nb;
-    sBt[f] = bt;
+    const int l = x / 6, f = x % 6, e = e0 + l;
+    int nb = 0, bt = -1;
+    if (e < mesh.nelem)
+    {
+      const int ci = e % mesh.nx, cj = (e / mesh.nx) % mesh.ny, ck = e / (mesh.nx * mesh.ny);
+      nb = neighbour(mesh, ci, cj, ck, f, &bt);
+      if (nb < 0)
+        nb = -1 - bface_index(mesh, ci, cj, ck, f);
+    }
+    sNb[l][f] = nb;
+    sBt[l][f] = bt;
   }
 }
 
-// Copies the tables of (p, q) and the element's geometry nodes to LDS.
+// Copies the tables of (p, q) and the geometry nodes of the epb elements
+// from e0 to LDS (sX[l * (q+1)^3 * 3 ...]).
 template <int P>
-__device__ __forceinline__ void gen_load_tables_nodes(const MeshArgs& mesh, int q, int e,
-                                                      GenTab* sT, double* sX)
+__device__ __forceinline__ void gen_load_tables_nodes(const MeshArgs& mesh, int q, int e0,
+                                                      int epb, GenTab* sT, double* sX)
 {
   constexpr int BT = GDims<P>::BT;
   const double* src = reinterpret_cast<const double*>(&g_gtab[P * 8 + q - 1]);
@@ -136,158 +159,180 @@ __device__ __forceinline__ void gen_load_tables_nodes(const MeshArgs& mesh, int 
   for (int x = threadIdx.x; x < (int)(sizeof(GenTab) / sizeof(double)); x += BT)
     dst[x] = src[x];
   const int nq = q + 1, n3 = nq * nq * nq * 3;
-  const double* X = mesh.nodes + (size_t)e * n3;
-  for (int x = threadIdx.x; x < n3; x += BT)
-    sX[x] = X[x];
+  for (int x = threadIdx.x; x < epb * n3; x += BT)
+  {
+    const int l = x / n3, e = e0 + l;
+    sX[x] = (e < mesh.nelem) ? mesh.nodes[(size_t)e * n3 + (x - l * n3)] : 0.0;
+  }
 }
 
-// Face trace records of element e's state sV (see produce_traces in
-// dgx_stage.hpp for the layout): the trace U (5) and the gradient traction
-// (3) at every face point, and the boundary state on domain-boundary faces.
-// Every lane calls; loops over the 6 (p+1)^2 face points in strides of the
-// workgroup.
+// Face trace records of the epb elements from e0 whose states are in sV (see
+// produce_traces in dgx_stage.hpp for the layout): the trace U (5) and the
+// gradient traction (3) at every face point, and the boundary state on
+// domain-boundary faces; in passes of FPP whole faces, one face point per
+// lane.  Every lane calls.
 template <int P, typename TT>
 __device__ __forceinline__ void gen_produce_traces(const MeshArgs& mesh, const Phys& ph,
-                                                   const GenTab& t, const double (*sV)[GDims<P>::NB],
-                                                   const double* sX, int nq, const int* sNb,
-                                                   const int* sBt, double* sR, int e,
-                                                   TT* __restrict__ Tout, int* status)
+                                                   const GenTab& t,
+                                                   const double (*sV)[5][GDims<P>::NB],
+                                                   const double* sX, int nq, const int (*sNb)[6],
+                                                   const int (*sBt)[6], double* sR, int e0,
+                                                   int epb, TT* __restrict__ Tout, int* status)
 {
   using D = GDims<P>;
-  constexpr int N = D::N, NF = D::NF, BT = D::BT, TS = D::TS;
-  // traces (5) and normal derivatives (4) of every face point
-  for (int x = threadIdx.x; x < 6 * NF; x += BT)
-  {
-    const int f = x / NF, fpt = x % NF;
-    const int a = fpt % N, b = fpt / N, nd = f >> 1, hi = f & 1;
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-    {
-      double v = 0.0, dv = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m)
-      {
-        const double s = sV[r][line_node<N>(nd, m, a, b)];
-        v += t.L[hi][m] * s;
-        if (r < 4)
-          dv += t.Ld[hi][m] * s;
-      }
-      sR[x * TS + r] = v;
-      if (r < 4)
-        sR[x * TS + 5 + r] = dv;
-    }
-  }
-  __syncthreads();
+  constexpr int N = D::N, NF = D::NF, FPP = D::FPP, TS = D::TS;
+  const int n3 = nq * nq * nq * 3;
+  // with one element per workgroup the pass count is a compile-time constant
+  const int nfaces = (D::EPB == 1) ? 6 : epb * 6;
+  const int npass  = (D::EPB == 1) ? D::NPASS1 : (nfaces + FPP - 1) / FPP;
+  const int tid = threadIdx.x, slot = tid / NF, fpt = tid % NF;
+  const int a = fpt % N, b = fpt / N, fbase = slot * NF;
   TT* Tb = Tout + (size_t)mesh.nelem * 6 * 8 * NF;
   bool ok = true;
-  for (int x = threadIdx.x; x < 6 * NF; x += BT)
+  for (int pass = 0; pass < npass; ++pass)
   {
-    const int f = x / NF, fpt = x % NF;
-    const int a = fpt % N, b = fpt / N, nd = f >> 1, hi = f & 1;
-    const int fb = f * NF;
-    double U[5];
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-      U[r] = sR[x * TS + r];
-    double Ji[3][3], nv[3];
-    gen_face_geometry(t, sX, nq, nd, hi, a, b, Ji, nv);
-    double gr[3][4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
+    const int g = pass * FPP + slot, l = g / 6, f = g % 6, e = e0 + l;
+    const bool fact = (slot < FPP) && (g < nfaces) && (e < mesh.nelem);
+    const int nd = f >> 1, hi = f & 1;
+    // traces (5) and normal derivatives (4) of this pass's face points
+    if (fact)
     {
-      double d1 = 0.0, d2 = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m)
-      {
-        d1 += t.Dg[a][m] * sR[(fb + m + N * b) * TS + r];
-        d2 += t.Dg[b][m] * sR[(fb + a + N * m) * TS + r];
-      }
-      const double dr = sR[x * TS + 5 + r];
-      gr[0][r] = pick3(nd, dr, d1, d1);
-      gr[1][r] = pick3(nd, d1, dr, d2);
-      gr[2][r] = pick3(nd, d2, d2, dr);
-    }
-    double vg[3];
-    grad_traction(gr, U, Ji, nv, ph.mu, vg);
-    TT* T = Tout + ((size_t)e * 6 + f) * 8 * NF + fpt;
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-      T[r * NF] = (TT)U[r];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-      T[(5 + i) * NF] = (TT)vg[i];
-    const int nb = sNb[f];
-    if (nb < 0)
-    {
-      double Ub[5];
-      ok = boundary_state(sBt[f], U, nv, ph, Ub) && ok;
-      TT* B = Tb + (size_t)(-1 - nb) * 5 * NF + fpt;
 #pragma unroll
       for (int r = 0; r < 5; ++r)
-        B[r * NF] = (TT)Ub[r];
+      {
+        double v = 0.0, dv = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+        {
+          const double x = sV[l][r][line_node<N>(nd, m, a, b)];
+          v += t.L[hi][m] * x;
+          if (r < 4)
+            dv += t.Ld[hi][m] * x;
+        }
+        sR[tid * TS + r] = v;
+        if (r < 4)
+          sR[tid * TS + 5 + r] = dv;
+      }
     }
+    __syncthreads();
+    if (fact)
+    {
+      double U[5];
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+        U[r] = sR[tid * TS + r];
+      double Ji[3][3], nv[3];
+      gen_face_geometry(t, sX + l * n3, nq, nd, hi, a, b, Ji, nv);
+      double gr[3][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+        {
+          d1 += t.Dg[a][m] * sR[(fbase + m + N * b) * TS + r];
+          d2 += t.Dg[b][m] * sR[(fbase + a + N * m) * TS + r];
+        }
+        const double dr = sR[tid * TS + 5 + r];
+        gr[0][r] = pick3(nd, dr, d1, d1);
+        gr[1][r] = pick3(nd, d1, dr, d2);
+        gr[2][r] = pick3(nd, d2, d2, dr);
+      }
+      double vg[3];
+      grad_traction(gr, U, Ji, nv, ph.mu, vg);
+      TT* T = Tout + ((size_t)e * 6 + f) * 8 * NF + fpt;
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+        T[r * NF] = (TT)U[r];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        T[(5 + i) * NF] = (TT)vg[i];
+      const int nb = sNb[l][f];
+      if (nb < 0)
+      {
+        double Ub[5];
+        ok = boundary_state(sBt[l][f], U, nv, ph, Ub) && ok;
+        TT* B = Tb + (size_t)(-1 - nb) * 5 * NF + fpt;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          B[r * NF] = (TT)Ub[r];
+      }
+    }
+    __syncthreads();
   }
   if (!ok)
     atomicOr(status, 1);
 }
 
-// Trace records of a Gauss-nodal state V: one workgroup per element.
+// Trace records of a Gauss-nodal state V: epb elements per workgroup.
 template <int P, typename TT>
 __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_trace_kernel(MeshArgs mesh, Phys ph,
                                                                     const double* __restrict__ V,
                                                                     TT* __restrict__ Tout,
-                                                                    int* status, int q)
+                                                                    int* status, int q, int epb)
 {
   using D = GDims<P>;
-  constexpr int NB = D::NB;
-  __shared__ double sV[5][NB];
+  constexpr int NB = D::NB, EPB = D::EPB;
+  __shared__ double sV[EPB][5][NB];
   __shared__ double sR[D::RG];
   __shared__ GenTab sT;
-  __shared__ int sNb[6], sBt[6];
-  extern __shared__ double sX[];  // [(q+1)^3 * 3]
-  const int e = blockIdx.x, tid = threadIdx.x;
-  gen_load_tables_nodes<P>(mesh, q, e, &sT, sX);
-  gen_face_table(mesh, e, sNb, sBt);
-  if (tid < NB)
+  __shared__ int sNb[EPB][6], sBt[EPB][6];
+  extern __shared__ double sX[];  // [epb][(q+1)^3 * 3]
+  const int ep  = (EPB == 1) ? 1 : epb;
+  const int tid = threadIdx.x, e0 = blockIdx.x * ep;
+  const int le  = (EPB == 1) ? 0 : tid / NB;
+  const int nd0 = (EPB == 1) ? tid : tid % NB;
+  const int e   = e0 + le;
+  gen_load_tables_nodes<P>(mesh, q, e0, ep, &sT, sX);
+  gen_face_table(mesh, e0, ep, sNb, sBt);
+  if ((EPB == 1 ? tid < NB : le < ep) && e < mesh.nelem)
 #pragma unroll
     for (int r = 0; r < 5; ++r)
-      sV[r][tid] = V[((size_t)e * 5 + r) * NB + tid];
+      sV[le][r][nd0] = V[((size_t)e * 5 + r) * NB + nd0];
   __syncthreads();
-  gen_produce_traces<P, TT>(mesh, ph, sT, sV, sX, q + 1, sNb, sBt, sR, e, Tout, status);
+  gen_produce_traces<P, TT>(mesh, ph, sT, sV, sX, q + 1, sNb, sBt, sR, e0, ep, Tout, status);
 }
 
-// One right-hand-side evaluation f_g of one element fused with the RK stage
-// combination of StageArgs (see stage_body_lds in dgx_stage.hpp: the same
-// phases and face-point physics with a workgroup per element): volume terms,
+// One right-hand-side evaluation f_g of the epb elements from blockIdx.x *
+// epb, fused with the RK stage combination of StageArgs (see stage_body_lds
+// in dgx_stage.hpp: the same phases and face-point physics): volume terms,
 // face terms from the trace records Tin, the epilogue, and the trace records
 // of Vout into Tout.
 template <int P, typename TT>
 __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs mesh, Phys ph,
                                                                     StageArgs sa,
                                                                     const double* __restrict__ lam,
-                                                                    int* status, int q)
+                                                                    int* status, int q, int epb)
 {
   using D = GDims<P>;
-  constexpr int N = D::N, NB = D::NB, NF = D::NF, FS = D::FS, FPP = D::FPP;
-  __shared__ double sV[5][NB];
+  constexpr int N = D::N, NB = D::NB, NF = D::NF, FS = D::FS, FPP = D::FPP, EPB = D::EPB;
+  __shared__ double sV[EPB][5][NB];
   __shared__ double sR[D::RG];
   __shared__ GenTab sT;
-  __shared__ int sNb[6], sBt[6];
-  extern __shared__ double sX[];  // [(q+1)^3 * 3]
-  const int e = blockIdx.x, tid = threadIdx.x, nq = q + 1;
+  __shared__ int sNb[EPB][6], sBt[EPB][6];
+  extern __shared__ double sX[];  // [epb][(q+1)^3 * 3]
+  const int tid = threadIdx.x, nq = q + 1, n3 = nq * nq * nq * 3;
+  // this lane's node: node nd0 of element le (le < epb <= EPB); with one
+  // element per workgroup every index below is the compile-time form
+  const int ep  = (EPB == 1) ? 1 : epb;
+  const int e0  = blockIdx.x * ep;
+  const int le  = (EPB == 1) ? 0 : tid / NB;
+  const int nd0 = (EPB == 1) ? tid : tid % NB;
+  const int e   = e0 + le;
   const TT* __restrict__ Tin = reinterpret_cast<const TT*>(sa.Tin);
-  gen_load_tables_nodes<P>(mesh, q, e, &sT, sX);
-  gen_face_table(mesh, e, sNb, sBt);
-  const bool vact = tid < NB;
+  gen_load_tables_nodes<P>(mesh, q, e0, ep, &sT, sX);
+  gen_face_table(mesh, e0, ep, sNb, sBt);
+  const bool vact = (EPB == 1) ? (tid < NB && e0 < mesh.nelem) : (le < ep && e < mesh.nelem);
   if (vact)
 #pragma unroll
     for (int r = 0; r < 5; ++r)
-      sV[r][tid] = sa.Vin[((size_t)e * 5 + r) * NB + tid];
+      sV[le][r][nd0] = sa.Vin[((size_t)e * 5 + r) * NB + nd0];
   __syncthreads();
   const GenTab& t = sT;
 
   // ---- volume: gradient, fluxes ----
-  const int i0 = tid % N, i1 = (tid / N) % N, i2 = tid / (N * N);
+  const int i0 = nd0 % N, i1 = (nd0 / N) % N, i2 = nd0 / (N * N);
   double wJ = 1.0;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   double Ji[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
@@ -295,14 +340,14 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
   if (vact)
   {
     double J[3][3];
-    gen_jac_at(t, sX, nq, -1, 0, i0, i1, i2, J);
+    gen_jac_at(t, sX + le * n3, nq, -1, 0, i0, i1, i2, J);
     const double det = det3(J);
     inv3(J, det, Ji);
     wJ = t.w[i0] * t.w[i1] * t.w[i2] * det;
     double Vq[5];
 #pragma unroll
     for (int r = 0; r < 5; ++r)
-      Vq[r] = sV[r][tid];
+      Vq[r] = sV[le][r][nd0];
     double gr[3][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
@@ -311,9 +356,9 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
 #pragma unroll
       for (int m = 0; m < N; ++m)
       {
-        g0 += t.Dg[i0][m] * sV[r][nid<N>(m, i1, i2)];
-        g1 += t.Dg[i1][m] * sV[r][nid<N>(i0, m, i2)];
-        g2 += t.Dg[i2][m] * sV[r][nid<N>(i0, i1, m)];
+        g0 += t.Dg[i0][m] * sV[le][r][nid<N>(m, i1, i2)];
+        g1 += t.Dg[i1][m] * sV[le][r][nid<N>(i0, m, i2)];
+        g2 += t.Dg[i2][m] * sV[le][r][nid<N>(i0, i1, m)];
       }
       gr[0][r] = g0;
       gr[1][r] = g1;
@@ -329,7 +374,8 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
     if (vact)
 #pragma unroll
       for (int r = 0; r < 5; ++r)
-        sR[r * NB + tid] = wJ * (Ji[c][0] * H[0][r] + Ji[c][1] * H[1][r] + Ji[c][2] * H[2][r]);
+        sR[(le * 5 + r) * NB + nd0] =
+            wJ * (Ji[c][0] * H[0][r] + Ji[c][1] * H[1][r] + Ji[c][2] * H[2][r]);
     __syncthreads();
     if (vact)
     {
@@ -342,7 +388,7 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
         for (int m = 0; m < N; ++m)
         {
           const int nn = (c == 0) ? nid<N>(m, i1, i2) : ((c == 1) ? nid<N>(i0, m, i2) : nid<N>(i0, i1, m));
-          s += t.Dg[m][ic] * sR[r * NB + nn];
+          s += t.Dg[m][ic] * sR[(le * 5 + r) * NB + nn];
         }
         acc[r] += s;
       }
@@ -350,22 +396,26 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
     __syncthreads();
   }
 
-  // ---- faces, in passes of FPP whole faces (face point layout in sR as in
-  // the tuned kernel: G 0-4, normal contravariant 5-8, tangential 9-16) ----
-  for (int pass = 0; pass < D::NPASS; ++pass)
+  // ---- faces, in passes of FPP whole faces of the workgroup's elements
+  // (face point layout in sR as in the tuned kernel: G 0-4, normal
+  // contravariant 5-8, tangential 9-16) ----
+  const int nfaces = ep * 6;
+  const int npass  = (EPB == 1) ? D::NPASS1 : (nfaces + FPP - 1) / FPP;
+  for (int pass = 0; pass < npass; ++pass)
   {
     const int slot = tid / NF;
-    const int f    = pass * FPP + slot;
+    const int g    = pass * FPP + slot;  // face g % 6 of element g / 6
     const int fpt  = tid % NF;
     const int a = fpt % N, b = fpt / N;
-    const bool fact = (slot < FPP) && (f < 6);
+    const int l = g / 6, f = g % 6, ee = e0 + l;
+    const bool fact = (slot < FPP) && (g < nfaces) && (ee < mesh.nelem);
     const int nd = f >> 1, hi = f & 1;
     const int fbase = slot * NF;
     FacePoint fpo;
     if (fact)
     {
-      const int en = sNb[f];
-      const TT* To = Tin + ((size_t)e * 6 + f) * 8 * NF + fpt;
+      const int en = sNb[l][f];
+      const TT* To = Tin + ((size_t)ee * 6 + f) * 8 * NF + fpt;
       double UL[5], vgL[3];
 #pragma unroll
       for (int r = 0; r < 5; ++r)
@@ -374,8 +424,8 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
       for (int i = 0; i < 3; ++i)
         vgL[i] = To[(5 + i) * NF];
       double JiL[3][3], nv[3];
-      const double fJ = gen_face_geometry(t, sX, nq, nd, hi, a, b, JiL, nv);
-      const double lamL = lam[((size_t)e * 6 + f) * NF + fpt];
+      const double fJ = gen_face_geometry(t, sX + l * n3, nq, nd, hi, a, b, JiL, nv);
+      const double lamL = lam[((size_t)ee * 6 + f) * NF + fpt];
       double Ub[5], vgR[3] = {0.0, 0.0, 0.0};
       double lamR = 0.0;
       if (en >= 0)
@@ -451,13 +501,14 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
 #pragma unroll
       for (int ff = 0; ff < 6; ++ff)
       {
-        if (ff / FPP != pass)
+        const int gg = le * 6 + ff;
+        if (gg / FPP != pass)
           continue;
         const int fnd = ff >> 1, fhi = ff & 1;
         const int m  = pick3i(fnd, i0, i1, i2);
         const int t1 = (fnd == 0) ? i1 : i0;                   // first tangential index
         const int t2 = (fnd == 2) ? i1 : i2;                   // second tangential index
-        const int fp = (ff % FPP) * NF + t2 * N + t1;
+        const int fp = (gg % FPP) * NF + t2 * N + t1;
         acc[0] += t.L[fhi][m] * sR[fp * FS];
 #pragma unroll
         for (int r = 1; r < 5; ++r)
@@ -471,7 +522,7 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
   if (vact)
   {
     const double iwJ = rcp(wJ);
-    const size_t base = (size_t)e * 5 * NB + tid;
+    const size_t base = (size_t)e * 5 * NB + nd0;
 #pragma unroll
     for (int r = 0; r < 5; ++r)
     {
@@ -492,9 +543,9 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
         if (sa.ca != 0.0)
           o += sa.ca * sa.Vbase[ix];
         if (sa.cb != 0.0)
-          o += sa.cb * sV[r][tid];
+          o += sa.cb * sV[le][r][nd0];
         sa.Vout[ix] = o;
-        sV[r][tid] = o;  // only this lane reads this node from here on
+        sV[le][r][nd0] = o;  // only this lane reads this node from here on
       }
     }
   }
@@ -503,7 +554,7 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_stage_kernel(MeshArgs me
   if (sa.Tout)
   {
     __syncthreads();
-    gen_produce_traces<P, TT>(mesh, ph, t, sV, sX, nq, sNb, sBt, sR, e,
+    gen_produce_traces<P, TT>(mesh, ph, t, sV, sX, nq, sNb, sBt, sR, e0, ep,
                               reinterpret_cast<TT*>(sa.Tout), status);
   }
 }
@@ -544,8 +595,9 @@ __global__ __launch_bounds__(256) void dgx_gen_lift_coef_kernel(MeshArgs mesh, d
   lam[x] = fJ * s;
 }
 
-// Applies a 1D matrix along direction dir (M^T with trans) to the element's
-// five variables in s; every lane calls (two barriers).
+// Applies a 1D matrix along direction dir (M^T with trans) to the five
+// variables of the lane's element s (node = this lane's node); every lane
+// calls (two barriers).
 template <int N>
 __device__ __forceinline__ void gen_apply_dir(double (*s)[N * N * N], const double (*M)[GMAXN],
                                               int dir, bool trans, int node, bool act)
@@ -640,44 +692,52 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_basis_kernel(const doubl
       out[(e * 5 + r) * NB + node] = sh[r][node] + sl[r][node];
 }
 
-// From f_g of one element per workgroup: f = B3^-1 f_g, R = -B3^T W f_g
-// (either may be null) and, with partial, partial[e] = sum of R^2 over the
-// element (waves summed in a fixed order: deterministic).
+// From f_g of the epb elements from blockIdx.x * epb: f = B3^-1 f_g,
+// R = -B3^T W f_g (either may be null) and, with partial,
+// partial[blockIdx.x] = sum of R^2 over those elements (waves summed in a
+// fixed order: deterministic).
 template <int P>
 __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_resid_out_kernel(MeshArgs mesh,
                                                                         const double* fg,
                                                                         double* f, double* R,
-                                                                        double* partial, int q)
+                                                                        double* partial, int q,
+                                                                        int epb)
 {
   using D = GDims<P>;
-  constexpr int N = D::N, NB = D::NB, BT = D::BT;
+  constexpr int N = D::N, NB = D::NB, BT = D::BT, EPB = D::EPB;
   const GenTab& t = g_gtab[P * 8 + q - 1];
-  __shared__ double s[5][NB];
+  __shared__ double s[EPB][5][NB];
   __shared__ double red[BT / 64];
-  extern __shared__ double sX[];  // [(q+1)^3 * 3]
-  const int node = threadIdx.x, nq = q + 1, n3 = nq * nq * nq * 3;
-  const size_t e = blockIdx.x;
-  const bool act = node < NB;
-  for (int x = node; x < n3; x += BT)
-    sX[x] = mesh.nodes[e * n3 + x];
+  extern __shared__ double sX[];  // [epb][(q+1)^3 * 3]
+  const int tid = threadIdx.x, nq = q + 1, n3 = nq * nq * nq * 3;
+  const int e0 = blockIdx.x * epb;
+  const int le = tid / NB, node = tid % NB;
+  const size_t e = (size_t)e0 + le;
+  const bool act = le < epb && e < (size_t)mesh.nelem;
+  double (*se)[NB] = s[act ? le : 0];
+  for (int x = tid; x < epb * n3; x += BT)
+  {
+    const int l = x / n3;
+    sX[x] = (e0 + l < mesh.nelem) ? mesh.nodes[(size_t)(e0 + l) * n3 + (x - l * n3)] : 0.0;
+  }
   double fgv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (act)
 #pragma unroll
     for (int r = 0; r < 5; ++r)
     {
       fgv[r] = fg[(e * 5 + r) * NB + node];
-      s[r][node] = fgv[r];
+      se[r][node] = fgv[r];
     }
   __syncthreads();
   if (f)
   {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-      gen_apply_dir<N>(s, t.Binv, d, false, node, act);
+      gen_apply_dir<N>(se, t.Binv, d, false, node, act);
     if (act)
 #pragma unroll
       for (int r = 0; r < 5; ++r)
-        f[(e * 5 + r) * NB + node] = s[r][node];
+        f[(e * 5 + r) * NB + node] = se[r][node];
     __syncthreads();
   }
   double sq = 0.0;
@@ -687,21 +747,21 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_resid_out_kernel(MeshArg
     if (act)
     {
       double J[3][3];
-      gen_jac_at(t, sX, nq, -1, 0, i0, i1, i2, J);
+      gen_jac_at(t, sX + le * n3, nq, -1, 0, i0, i1, i2, J);
       const double wJ = t.w[i0] * t.w[i1] * t.w[i2] * det3(J);
 #pragma unroll
       for (int r = 0; r < 5; ++r)
-        s[r][node] = -wJ * fgv[r];
+        se[r][node] = -wJ * fgv[r];
     }
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-      gen_apply_dir<N>(s, t.B, d, true, node, act);
+      gen_apply_dir<N>(se, t.B, d, true, node, act);
     if (act)
 #pragma unroll
       for (int r = 0; r < 5; ++r)
       {
-        const double v = s[r][node];
+        const double v = se[r][node];
         if (R)
           R[(e * 5 + r) * NB + node] = v;
         sq += v * v;
@@ -712,15 +772,15 @@ __global__ __launch_bounds__(GDims<P>::BT) void dgx_gen_resid_out_kernel(MeshArg
 #pragma unroll
     for (int w = 32; w > 0; w >>= 1)
       sq += __shfl_xor(sq, w, 64);
-    if ((node & 63) == 0)
-      red[node >> 6] = sq;
+    if ((tid & 63) == 0)
+      red[tid >> 6] = sq;
     __syncthreads();
-    if (node == 0)
+    if (tid == 0)
     {
       double tot = 0.0;
       for (int w = 0; w < BT / 64; ++w)
         tot += red[w];
-      partial[e] = tot;
+      partial[blockIdx.x] = tot;
     }
   }
 }
@@ -807,22 +867,35 @@ static bool gen_orders_ok(int p, int q)
   return p >= 0 && p <= 8 && q >= 1 && q <= 8;
 }
 
-static size_t gen_node_lds(int q)
+// dynamic LDS of epb elements' geometry nodes
+static size_t gen_node_lds(int q, int epb)
 {
-  return (size_t)(q + 1) * (q + 1) * (q + 1) * 3 * sizeof(double);
+  return (size_t)epb * (q + 1) * (q + 1) * (q + 1) * 3 * sizeof(double);
+}
+
+static const int g_gen_epb_max[9] = {GDims<0>::EPB, GDims<1>::EPB, GDims<2>::EPB,
+                                     GDims<3>::EPB, GDims<4>::EPB, GDims<5>::EPB,
+                                     GDims<6>::EPB, GDims<7>::EPB, GDims<8>::EPB};
+
+int gen_elements_per_block(int p, int q)
+{
+  if (!gen_orders_ok(p, q))
+    return 1;
+  return gen_epb_of(g_gen_epb_max[p], q);
 }
 
 template <int P>
 static hipError_t launch_gen_rhs(int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
                                  const double* lam, int* status, hipStream_t st)
 {
-  const dim3 grid(m.nelem), block(GDims<P>::BT);
+  const int epb = gen_epb_of(GDims<P>::EPB, q);
+  const dim3 grid((m.nelem + epb - 1) / epb), block(GDims<P>::BT);
   if (sa.trace32)
-    hipLaunchKernelGGL((dgx_gen_stage_kernel<P, float>), grid, block, gen_node_lds(q), st, m, ph,
-                       sa, lam, status, q);
+    hipLaunchKernelGGL((dgx_gen_stage_kernel<P, float>), grid, block, gen_node_lds(q, epb), st,
+                       m, ph, sa, lam, status, q, epb);
   else
-    hipLaunchKernelGGL((dgx_gen_stage_kernel<P, double>), grid, block, gen_node_lds(q), st, m,
-                       ph, sa, lam, status, q);
+    hipLaunchKernelGGL((dgx_gen_stage_kernel<P, double>), grid, block, gen_node_lds(q, epb), st,
+                       m, ph, sa, lam, status, q, epb);
   return hipGetLastError();
 }
 
@@ -840,13 +913,14 @@ template <int P>
 static hipError_t launch_gen_trace(int q, const MeshArgs& m, const Phys& ph, const double* V,
                                    double* Tout, int* status, hipStream_t st, bool trace32)
 {
-  const dim3 grid(m.nelem), block(GDims<P>::BT);
+  const int epb = gen_epb_of(GDims<P>::EPB, q);
+  const dim3 grid((m.nelem + epb - 1) / epb), block(GDims<P>::BT);
   if (trace32)
-    hipLaunchKernelGGL((dgx_gen_trace_kernel<P, float>), grid, block, gen_node_lds(q), st, m, ph,
-                       V, reinterpret_cast<float*>(Tout), status, q);
+    hipLaunchKernelGGL((dgx_gen_trace_kernel<P, float>), grid, block, gen_node_lds(q, epb), st,
+                       m, ph, V, reinterpret_cast<float*>(Tout), status, q, epb);
   else
-    hipLaunchKernelGGL((dgx_gen_trace_kernel<P, double>), grid, block, gen_node_lds(q), st, m,
-                       ph, V, Tout, status, q);
+    hipLaunchKernelGGL((dgx_gen_trace_kernel<P, double>), grid, block, gen_node_lds(q, epb), st,
+                       m, ph, V, Tout, status, q, epb);
   return hipGetLastError();
 }
 
@@ -874,8 +948,9 @@ template <int P>
 static hipError_t launch_gen_resid_out(int q, const MeshArgs& m, const double* fg, double* f,
                                        double* R, double* partial, hipStream_t st)
 {
-  hipLaunchKernelGGL((dgx_gen_resid_out_kernel<P>), dim3(m.nelem), dim3(GDims<P>::BT),
-                     gen_node_lds(q), st, m, fg, f, R, partial, q);
+  const int epb = gen_epb_of(GDims<P>::EPB, q);
+  hipLaunchKernelGGL((dgx_gen_resid_out_kernel<P>), dim3((m.nelem + epb - 1) / epb),
+                     dim3(GDims<P>::BT), gen_node_lds(q, epb), st, m, fg, f, R, partial, q, epb);
   return hipGetLastError();
 }
 

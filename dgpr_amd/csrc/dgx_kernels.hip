@@ -679,11 +679,15 @@ size_t trace_doubles(int p, int nx, int ny, int nz)
   return (size_t)nx * ny * nz * 6 * 8 * nf + boundary_faces(nx, ny, nz) * 5 * nf;
 }
 
-// general orders: one workgroup (and one partial) per element
+// general orders: one partial per workgroup of gen_elements_per_block
+// elements
 int resid_out_blocks(int p, int q, int nelem)
 {
   if (general_order(p, q))
-    return nelem;
+  {
+    const int epb = gen_elements_per_block(p, q);
+    return (nelem + epb - 1) / epb;
+  }
   int nb = (p + 1) * (p + 1) * (p + 1);
   int epw = 64 / nb;
   return (nelem + epw - 1) / epw;
@@ -816,7 +820,7 @@ hipError_t lift_coef_dispatch(int p, int q, const MeshArgs& m, double* lam, hipS
 int norm_blocks(int p, int q, int nelem)
 {
   if (general_order(p, q))
-    return nelem;
+    return resid_out_blocks(p, q, nelem);
   const int epb = 4 * (64 / ((p + 1) * (p + 1) * (p + 1)));
   return (nelem + epb - 1) / epb;
 }

@@ -219,8 +219,11 @@ inline bool general_order(int p, int q)
 }
 // tables of every (p, q), 0 <= p <= 8, 1 <= q <= 8: gt[p * 8 + q - 1]
 hipError_t upload_gen_tables(const GenTab* gt);
+// elements per workgroup of the general-order kernels for (p, q) (several
+// at p <= 2, as many as 32 KB of their geometry nodes allow)
+int gen_elements_per_block(int p, int q);
 // the entry points below mirror the tuned ones for the general orders
-// (one workgroup per element, one Gauss node per lane)
+// (one Gauss node per lane; gen_elements_per_block elements per workgroup)
 hipError_t gen_rhs_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const StageArgs& sa,
                             const double* lam, int* status, hipStream_t st);
 hipError_t gen_trace_dispatch(int p, int q, const MeshArgs& m, const Phys& ph, const double* V,

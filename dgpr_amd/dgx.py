@@ -548,7 +548,7 @@ class Solver:
     def tile(self):
         """Stage tile: 0 (one row of elements per wavefront, mixed-direction
         face passes), 1 (tiled, one direction per face subpass) or 2 (the
-        general-order kernel, one workgroup per element: the only one
+        general-order kernel, one Gauss node per lane: the only one
         outside 1 <= p, q <= 3, and available at every order)."""
         t = C.c_int()
         _chk(lib().dgx_get_tile(self.ctx, C.byref(t)), self.ctx)

@@ -95,7 +95,7 @@ typedef struct dgx_ctx dgx_ctx;
  * 819-830).  1 <= p, q <= 3 run the tuned kernels; a q = 1 mesh there whose
  * elements are all parallelepipeds runs the constant-Jacobian kernels
  * (chosen here, nothing to set).  The other orders run the general-order
- * kernels (stage tile 2, one workgroup per element); the adjoint covers
+ * kernels (stage tile 2, one Gauss node per lane); the adjoint covers
  * 1 <= p, q <= 3 and returns DGX_ERR_UNSUPPORTED outside. */
 dgx_status dgx_create(const dgx_mesh* mesh, const dgx_params* params,
                       dgx_ctx** out);
@@ -387,8 +387,8 @@ dgx_status dgx_autotune(dgx_ctx* ctx, int reps, int* chosen, double us_out[3]);
  * owning one node per row, faces processed one reference direction at a
  * time in whole-face subpasses (compiled for q = 1 meshes at 2 and 3 waves
  * per SIMD; selecting it moves a target of 1 to 2; DGX_ERR_UNSUPPORTED for
- * q > 1); 2 runs the general-order kernel (one workgroup per element, one
- * Gauss node per lane; every order, and the only tile outside 1 <= p, q <= 3,
+ * q > 1); 2 runs the general-order kernel (one element, or several at
+ * p <= 2, per workgroup, one Gauss node per lane; every order, and the only tile outside 1 <= p, q <= 3,
  * where 0 and 1 return DGX_ERR_UNSUPPORTED; occupancy targets do not apply).
  * Results agree to rounding. */
 dgx_status dgx_set_tile(dgx_ctx* ctx, int tile);
