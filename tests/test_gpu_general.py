@@ -66,6 +66,11 @@ GENERAL = {
     "tgv_p0": lambda: cases.tgv_case(n=4, p=0, seed=93, M=0.2, nsteps=5),
     "channel_p0_inviscid": lambda: cases.channel_case(4, 4, 2, p=0, seed=94, nsteps=5,
                                                       wall=dgx.INVISCID_WALL),
+    # several elements per workgroup with a partly filled last workgroup:
+    # 18 elements in workgroups of 8 (p = 1, q = 4), 5 in workgroups of 2
+    # (p = 2, q = 5)
+    "channel_q4_p1": lambda: cases.channel_case(3, 3, 2, p=1, q=4, seed=95, nsteps=3),
+    "channel_q5_p2": lambda: cases.channel_case(5, 1, 1, p=2, q=5, seed=95, nsteps=3),
 }
 
 
